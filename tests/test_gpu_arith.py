@@ -1,0 +1,213 @@
+"""GPU parity: add, sub, mul, div, mod, divmod against GMP and the C oracle.
+
+Mirrors the reference's genericTwoInOneOutTest / SubTest / genericTwoInTwoOutTest
+(unit_tests/test.cu:1674-2376): seeded inputs, the device op through the
+C ABI, GMP on the host, and the low `limbs` words of every instance compared
+(test.cu:1223-1228).  Bit-exact.
+"""
+import numpy as np
+import pytest
+
+import xmp_amd
+from helpers import O, assert_rows_equal, gmp, inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(seed, count, la, lb):
+    return inputs.uniform(seed, count, la), inputs.uniform(seed + 7, count, lb)
+
+
+@pytest.mark.parametrize("bits_a,bits_b,bits_s", [(32, 32, 64), (64, 64, 64), (256, 128, 288), (2048, 2048, 2048),
+                                                  (2048, 2048, 2080), (2048, 1024, 4096), (4096, 4096, 4096)])
+def test_add(handle, bits_a, bits_b, bits_s):
+    n = 1000
+    a, b = _pair(11, n, bits_a // 32, bits_b // 32)
+    A, B = handle.from_array(a), handle.from_array(b)
+    S = handle.integers(bits_s, n)
+    handle.add(S, A, B)
+    assert_rows_equal(S.get(), gmp(O.GO_ADD, n, bits_s // 32, a, b), "add")
+
+
+@pytest.mark.parametrize("bits_a,bits_b,bits_d", [(64, 64, 64), (2048, 2048, 2048), (2048, 1024, 2048),
+                                                  (1024, 2048, 2048), (512, 512, 1024)])
+def test_sub_wraps(handle, bits_a, bits_b, bits_d):
+    n = 1000
+    a, b = _pair(12, n, bits_a // 32, bits_b // 32)
+    A, B = handle.from_array(a), handle.from_array(b)
+    D = handle.integers(bits_d, n)
+    handle.sub(D, A, B)
+    assert_rows_equal(D.get(), gmp(O.GO_SUB, n, bits_d // 32, a, b), "sub")
+
+
+def test_add_sub_roundtrip_full_size(handle):
+    """Size-independent property at the benchmark shape: (a+b)-b == a (mod 2^P)."""
+    n = 1 << 20
+    a, b = inputs.ordered_pair(21, n, 64)
+    A, B = handle.from_array(a), handle.from_array(b)
+    S, D = handle.integers(2048, n), handle.integers(2048, n)
+    handle.add(S, A, B)
+    handle.sub(D, S, B)
+    assert np.array_equal(D.get(), a)
+    handle.sub(D, A, B)
+    sample = np.arange(0, n, 997)
+    assert_rows_equal(D.get()[sample], gmp(O.GO_SUB, sample.size, 64, a[sample], b[sample]), "sub sample")
+
+
+def test_add_carry_extremes(handle):
+    n = 64
+    ones = np.full((n, 64), 0xFFFFFFFF, np.uint32)
+    one = np.zeros((n, 64), np.uint32)
+    one[:, 0] = 1
+    A, B = handle.from_array(ones), handle.from_array(one)
+    S = handle.integers(2080, n)
+    handle.add(S, A, B)
+    want = np.zeros((n, 65), np.uint32)
+    want[:, 64] = 1
+    assert np.array_equal(S.get(), want)
+    D = handle.integers(2048, n)
+    handle.sub(D, B, A)   # 1 - (2^2048-1) wraps to 2
+    want = np.zeros((n, 64), np.uint32)
+    want[:, 0] = 2
+    assert np.array_equal(D.get(), want)
+
+
+def test_add_broadcast_and_indices(handle):
+    n, m = 512, 17
+    a = inputs.uniform(31, n, 8)
+    b = inputs.uniform(32, m, 8)
+    A, B = handle.from_array(a), handle.from_array(b)
+    S = handle.integers(256, n)
+    handle.add(S, A, B)   # b read as i % m
+    want = gmp(O.GO_ADD, n, 8, a, b)
+    assert_rows_equal(S.get(), want, "broadcast")
+    rng = np.random.default_rng(3)
+    ia = rng.permutation(n).astype(np.uint32)
+    ib = rng.integers(0, 1000, 100).astype(np.uint32)
+    io = rng.permutation(n).astype(np.uint32)
+    handle.set_indices(0, io)
+    handle.set_indices(1, ia)
+    handle.set_indices(2, ib)
+    try:
+        handle.add(S, A, B)
+        got = S.get()
+    finally:
+        handle.clear_indices()
+    want = np.zeros_like(got)
+    for i in range(n):
+        x = O.to_int(a[ia[i % n] % n]) + O.to_int(b[ib[i % 100] % m])
+        want[io[i]] = O.from_int(x % (1 << 256), 8)
+    assert_rows_equal(got, want, "indexed add")
+
+
+def test_add_in_place(handle):
+    n = 300
+    a, b = _pair(41, n, 16, 16)
+    A, B = handle.from_array(a), handle.from_array(b)
+    handle.add(A, A, B)
+    assert_rows_equal(A.get(), gmp(O.GO_ADD, n, 16, a, b), "in-place add")
+
+
+@pytest.mark.parametrize("bits_a,bits_b,bits_p", [(32, 32, 64), (128, 128, 256), (512, 512, 1024),
+                                                  (1024, 1024, 2048), (2048, 2048, 4096), (2048, 2048, 2048),
+                                                  (2048, 1024, 3072), (96, 2048, 2144), (4096, 4096, 8192)])
+def test_mul(handle, bits_a, bits_b, bits_p):
+    n = 500
+    a, b = _pair(51, n, bits_a // 32, bits_b // 32)
+    A, B = handle.from_array(a), handle.from_array(b)
+    P = handle.integers(bits_p, n)
+    handle.mul(P, A, B)
+    assert_rows_equal(P.get(), gmp(O.GO_MUL, n, bits_p // 32, a, b), "mul")
+
+
+def test_mul_closed_form_and_square(handle):
+    """(2^k - 1)^2 = 2^2k - 2^(k+1) + 1, and mul(a, a) takes the square path."""
+    n = 64
+    ones = np.full((n, 64), 0xFFFFFFFF, np.uint32)
+    A = handle.from_array(ones)
+    P = handle.integers(4096, n)
+    handle.mul(P, A, A)
+    k = 2048
+    want = O.from_int((1 << (2 * k)) - (1 << (k + 1)) + 1, 128)
+    assert all(np.array_equal(r, want) for r in P.get())
+    a = inputs.uniform(52, n, 64)
+    A = handle.from_array(a)
+    handle.mul(P, A, A)
+    assert_rows_equal(P.get(), gmp(O.GO_MUL, n, 128, a, a), "square")
+
+
+@pytest.mark.parametrize("bits_a,bits_b", [(64, 32), (128, 64), (512, 256), (1024, 1024), (4096, 2048),
+                                           (2048, 96), (8192, 4096), (3072, 3072)])
+def test_div_mod(handle, bits_a, bits_b):
+    n = 700
+    a = inputs.uniform(61, n, bits_a // 32)
+    b = inputs.uniform(62, n, bits_b // 32)
+    b[:, -1] |= np.uint32(0x80000000) if bits_b >= 64 else np.uint32(0)
+    b[b.sum(axis=1) == 0, 0] = 1
+    A, B = handle.from_array(a), handle.from_array(b)
+    Q, R = handle.integers(bits_a, n), handle.integers(bits_b, n)
+    handle.div(Q, A, B)
+    handle.mod(R, A, B)
+    wq, wr = gmp(O.GO_DIV, n, bits_a // 32, a, b, out2_limbs=bits_b // 32)
+    assert_rows_equal(Q.get(), wq, "div")
+    assert_rows_equal(R.get(), wr, "mod")
+    Q2, R2 = handle.integers(bits_a, n), handle.integers(bits_b, n)
+    handle.divmod(Q2, R2, A, B)
+    assert_rows_equal(Q2.get(), wq, "divmod q")
+    assert_rows_equal(R2.get(), wr, "divmod r")
+
+
+def test_div_short_and_unnormalised_divisors(handle):
+    """Divisors with leading zero limbs / bits, a < b, and b = 1."""
+    n = 800
+    a = inputs.uniform(71, n, 128)
+    b = inputs.uniform(72, n, 64)
+    rng = np.random.default_rng(9)
+    for i in range(n):
+        top = int(rng.integers(1, 65))
+        b[i, top:] = 0
+        b[i, top - 1] >>= int(rng.integers(0, 32))
+        if not b[i].any():
+            b[i, 0] = 1
+    a[: n // 8, 64:] = 0   # some a < b
+    b[n - 5:] = 0
+    b[n - 5:, 0] = 1
+    A, B = handle.from_array(a), handle.from_array(b)
+    Q, R = handle.integers(4096, n), handle.integers(2048, n)
+    handle.divmod(Q, R, A, B)
+    wq, wr = gmp(O.GO_DIV, n, 128, a, b, out2_limbs=64)
+    assert_rows_equal(Q.get(), wq, "div q")
+    assert_rows_equal(R.get(), wr, "div r")
+
+
+def test_div_by_zero_defined(handle):
+    n = 64
+    a = inputs.uniform(81, n, 8)
+    A = handle.from_array(a)
+    B = handle.from_array(np.zeros((n, 4), np.uint32))
+    Q, R = handle.integers(256, n), handle.integers(128, n)
+    handle.divmod(Q, R, A, B)
+    assert not Q.get().any()
+    assert np.array_equal(R.get(), a[:, :4])
+
+
+def test_div_invariant_full_size(handle):
+    """Config shape: 65,536 x (4096 / 2048) -- q*b + r == a and r < b on every instance."""
+    n = 65536
+    a = inputs.uniform(91, n, 128)
+    b = inputs.uniform(92, n, 64)
+    b[:, -1] |= np.uint32(0x80000000)
+    A, B = handle.from_array(a), handle.from_array(b)
+    Q, R = handle.integers(4096, n), handle.integers(2048, n)
+    handle.divmod(Q, R, A, B)
+    q, r = Q.get(), R.get()
+    QB = handle.integers(4096, n)
+    handle.mul(QB, Q, B)
+    S = handle.integers(4096, n)
+    Rw = handle.from_array(np.concatenate([r, np.zeros((n, 64), np.uint32)], axis=1))
+    handle.add(S, QB, Rw)
+    assert np.array_equal(S.get(), a)
+    sample = np.arange(0, n, 251)
+    wq, wr = gmp(O.GO_DIV, sample.size, 128, a[sample], b[sample], out2_limbs=64)
+    assert_rows_equal(q[sample], wq, "q sample")
+    assert_rows_equal(r[sample], wr, "r sample")

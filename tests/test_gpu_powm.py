@@ -1,0 +1,167 @@
+"""GPU parity: xmpIntegersPowm against GMP mpz_powm and the C oracle.
+
+Mirrors the reference's PowmTest (unit_tests/test.cu:1098-1270) and its
+instantiations over sizes and algorithms (test.cu:2815-2910): seeded odd
+moduli, bases not reduced below m, the device op through the C ABI, GMP on the
+host, every limb compared.  Bit-exact.
+"""
+import numpy as np
+import pytest
+
+import xmp_amd
+from helpers import O, assert_rows_equal, gmp, inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def _powm_case(handle, seed, n, bits, exp_bits=None, reduce_base=True, count_mod=None):
+    limbs = bits // 32
+    el = (exp_bits or bits) // 32
+    m = inputs.odd_moduli(seed, count_mod or n, limbs)
+    b = inputs.below(seed + 1, m) if reduce_base else inputs.uniform(seed + 1, n, limbs)
+    if b.shape[0] != n:
+        b = inputs.uniform(seed + 1, n, limbs)
+    e = inputs.uniform(seed + 2, n, el)
+    M, B, E = handle.from_array(m), handle.from_array(b), handle.from_array(e)
+    Out = handle.integers(bits, n)
+    handle.powm(Out, B, E, M)
+    return Out.get(), b, e, m
+
+
+@pytest.mark.parametrize("bits", [64, 128, 256, 320, 512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192])
+def test_powm_sizes(handle, bits):
+    n = 128 if bits <= 2048 else 64
+    got, b, e, m = _powm_case(handle, 100 + bits, n, bits)
+    want = gmp(O.GO_POWM, n, bits // 32, b, e, m)
+    assert_rows_equal(got, want, "powm %d" % bits)
+
+
+@pytest.mark.parametrize("bits", [1024, 2048])
+def test_powm_matches_c_oracle(handle, bits):
+    n = 16
+    got, b, e, m = _powm_case(handle, 7 + bits, n, bits)
+    want = O.powm_batch(b, e, m, n)
+    assert_rows_equal(got, want, "powm vs restated reference algorithm")
+
+
+@pytest.mark.parametrize("alg", [xmp_amd.ALG_REGMP, xmp_amd.ALG_DISTRIBUTEDMP])
+@pytest.mark.parametrize("bits", [512, 1024, 2048])
+def test_powm_forced_layouts(handle, alg, bits):
+    n = 96
+    handle.set_algorithm(alg)
+    try:
+        got, b, e, m = _powm_case(handle, 300 + bits, n, bits)
+    except xmp_amd.XmpError as err:
+        assert err.code == xmp_amd.UNSUPPORTED
+        return
+    finally:
+        handle.set_algorithm(xmp_amd.ALG_DEFAULT)
+    assert_rows_equal(got, gmp(O.GO_POWM, n, bits // 32, b, e, m), "forced layout")
+
+
+def test_powm_unreduced_base_and_short_exponent(handle):
+    n = 200
+    got, b, e, m = _powm_case(handle, 401, n, 2048, exp_bits=32, reduce_base=False)
+    assert_rows_equal(got, gmp(O.GO_POWM, n, 64, b, e, m), "base >= m, 32-bit exponent")
+    got, b, e, m = _powm_case(handle, 402, n, 1024, exp_bits=2048, reduce_base=False)
+    assert_rows_equal(got, gmp(O.GO_POWM, n, 32, b, e, m), "exponent longer than modulus")
+
+
+def test_powm_special_values(handle):
+    """e = 0 -> 1 (0 for m = 1); b = 0 -> 0; e = 1 -> b mod m; b = m - 1; m = 1; small m in wide precision."""
+    limbs = 32
+    n = 12
+    m = inputs.odd_moduli(501, n, limbs)
+    b = inputs.uniform(502, n, limbs)
+    e = inputs.uniform(503, n, limbs)
+    e[0] = 0
+    b[1] = 0
+    e[2] = 0
+    e[2, 0] = 1
+    b[3] = m[3]
+    b[3, 0] -= 1
+    m[4] = 0
+    m[4, 0] = 1               # m = 1
+    e[4] = 0                  # 0^0 mod 1 = 0
+    m[5] = 0
+    m[5, 0] = 0x10001         # small modulus in a 1024-bit container
+    m[6, 20:] = 0
+    m[6, 0] |= 1              # modulus with leading zero limbs
+    b[7] = m[7]               # b == m
+    e[8] = 0xFFFFFFFF         # all-ones exponent
+    M, B, E = handle.from_array(m), handle.from_array(b), handle.from_array(e)
+    Out = handle.integers(1024, n)
+    handle.powm(Out, B, E, M)
+    assert_rows_equal(Out.get(), gmp(O.GO_POWM, n, limbs, b, e, m), "special values")
+
+
+def test_powm_fermat_and_crt_sample_key(handle):
+    """Toy RSA key of the reference samples (samples/crt_rsa/rsa.cc:167-178):
+    n = 17460671 = 4931 * 3541, e = 65537, d = 16156673; m^(e*d) == m, and a
+    prime p gives a^(p-1) == 1 (Fermat)."""
+    msgs = np.array([[12345], [2], [17460670], [99999]], np.uint32)
+    N = handle.from_array(np.full((4, 1), 17460671, np.uint32))
+    E = handle.from_array(np.full((4, 1), 65537, np.uint32))
+    D = handle.from_array(np.full((4, 1), 16156673, np.uint32))
+    M = handle.from_array(msgs)
+    C, P = handle.integers(32, 4), handle.integers(32, 4)
+    handle.powm(C, M, E, N)
+    handle.powm(P, C, D, N)
+    assert np.array_equal(P.get(), msgs)
+    p = (1 << 127) - 1   # Mersenne prime
+    a = inputs.uniform(601, 32, 4)
+    pa = np.tile(O.from_int(p, 4), (32, 1))
+    pm1 = np.tile(O.from_int(p - 1, 4), (32, 1))
+    Out = handle.integers(128, 32)
+    handle.powm(Out, handle.from_array(a), handle.from_array(pm1), handle.from_array(pa))
+    got = Out.get()
+    for i in range(32):
+        want = 0 if O.to_int(a[i]) % p == 0 else 1
+        assert O.to_int(got[i]) == want
+
+
+def test_powm_indices_and_resident_moduli(handle):
+    """Policy index gather (slots 1..3) and scatter (slot 0), as the reference
+    powm kernels read them (powm_implementations.h:417-418, copy-out :480-516)."""
+    n, nm = 256, 16
+    m = inputs.odd_moduli(701, nm, 64)
+    b = inputs.uniform(702, n, 64)
+    e = inputs.uniform(703, 8, 64)
+    rng = np.random.default_rng(5)
+    im = rng.integers(0, 10 * nm, 64).astype(np.uint32)
+    ie = rng.integers(0, 8, 32).astype(np.uint32)
+    io = rng.permutation(n).astype(np.uint32)
+    M, B, E = handle.from_array(m), handle.from_array(b), handle.from_array(e)
+    Out = handle.integers(2048, n)
+    handle.set_indices(0, io)
+    handle.set_indices(2, ie)
+    handle.set_indices(3, im)
+    try:
+        handle.powm(Out, B, E, M)
+        got = Out.get()
+    finally:
+        handle.clear_indices()
+    bb = b
+    ee = e[ie[np.arange(n) % 32] % 8]
+    mm = m[im[np.arange(n) % 64] % nm]
+    want_rows = gmp(O.GO_POWM, n, 64, bb, ee, mm)
+    want = np.zeros_like(want_rows)
+    want[io] = want_rows
+    assert_rows_equal(got, want, "indexed powm")
+
+
+def test_powm_in_place(handle):
+    n = 100
+    m = inputs.odd_moduli(801, n, 32)
+    b = inputs.below(802, m)
+    e = inputs.uniform(803, n, 32)
+    M, B, E = handle.from_array(m), handle.from_array(b), handle.from_array(e)
+    handle.powm(B, B, E, M)
+    assert_rows_equal(B.get(), gmp(O.GO_POWM, n, 32, b, e, m), "in-place powm")
+
+
+def test_powm_config1_full(handle):
+    """4096 x 1024-bit (the reference's CPU-runnable configuration) against GMP."""
+    n = 4096
+    got, b, e, m = _powm_case(handle, 1001, n, 1024)
+    assert_rows_equal(got, gmp(O.GO_POWM, n, 32, b, e, m), "config 1")

@@ -1,0 +1,327 @@
+// ops.cpp -- math entry points: add, sub, mul, sqr, div, mod, divmod, powm.
+//
+// Validation order and error codes follow the reference (add_sub.cu:35-142,
+// mul.cu:26-303, div.cu:35-517, powm.cu:250-307).  In-place operation (the
+// output aliases an input) goes through the handle's scratch buffer as in the
+// reference; the reference's in-place DivMod stores its scratch pointers in
+// the index fields (div.cu:429-435) -- here it uses scratch outputs.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "internal.hpp"
+#include "kernels.hpp"
+#include "powm.hpp"
+#include "tuner.hpp"
+
+using namespace xmp;
+
+namespace {
+
+inline size_t out_bytes(xmpIntegers_t x) { return sizeof(uint32_t) * (size_t)x->nlimbs * x->stride; }
+
+// Common prologue of the two-input one-output ops.
+xmpError_t check_binary(xmpHandle_t h, xmpIntegers_t o, xmpIntegers_t a, xmpIntegers_t b, uint32_t count) {
+  if (o->device != h->device || a->device != h->device || b->device != h->device) return xmpErrorInvalidDevice;
+  return xmpErrorSuccess;
+}
+
+xmpError_t finish(xmpHandle_t h, bool inplace, xmpIntegers_t o, uint32_t *buffer) {
+  if (inplace &&
+      hipMemcpyAsync(o->limbs, buffer, out_bytes(o), hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+    return xmpErrorCuda;
+  XMP_CHECK_HIP();
+  return xmpErrorSuccess;
+}
+
+xmpError_t sync(xmpHandle_t h, xmpError_t e) {
+  if (e != xmpErrorSuccess) return e;
+  XMP_SET_DEVICE(h);
+  (void)hipStreamSynchronize(h->stream);
+  XMP_CHECK_HIP();
+  return xmpErrorSuccess;
+}
+
+enum class Bin { Add, Sub, Mul };
+
+xmpError_t binary_async(Bin op, xmpHandle_t h, xmpIntegers_t o, xmpIntegers_t a, xmpIntegers_t b, uint32_t count) {
+  xmpExecutionPolicy_t p = h->policy;
+  xmpError_t e = check_binary(h, o, a, b, count);
+  if (e != xmpErrorSuccess) return e;
+  if (count == 0) return xmpErrorSuccess;
+  XMP_SET_DEVICE(h);
+  if (o->count < count) return xmpErrorInvalidCount;
+  if (p->indices[0] && p->indices_count[0] < count) return xmpErrorInvalidCount;
+  if (o->precision < std::max(a->precision, b->precision)) return xmpErrorInvalidPrecision;
+  XMP_CHECK_HIP();
+  bool inplace = (o == a || o == b);
+  uint32_t *buffer = o->limbs;
+  if (inplace) {
+    e = set_necessary_scratch(h, out_bytes(o));
+    if (e != xmpErrorSuccess) return e;
+    buffer = (uint32_t *)h->scratch;
+    // indexed outputs leave unselected instances untouched: start from o
+    if (p->indices[0] &&
+        hipMemcpyAsync(buffer, o->limbs, out_bytes(o), hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+      return xmpErrorCuda;
+  }
+  OutOperand out = make_out(o, buffer, p, 0);
+  Operand A = make_in(a, p, 1), B = make_in(b, p, 2);
+  hipError_t st = hipSuccess;
+  switch (op) {
+    case Bin::Add: st = launch_add(out, A, B, count, h->stream); break;
+    case Bin::Sub: st = launch_sub(out, A, B, count, h->stream); break;
+    case Bin::Mul:
+      if (A.len < B.len) std::swap(A, B);   // reference mul.cu:168-175 orders by precision
+      st = launch_mul(out, A, B, count, h->stream);
+      break;
+  }
+  if (st != hipSuccess) return xmpErrorCuda;
+  return finish(h, inplace, o, buffer);
+}
+
+}  // namespace
+
+extern "C" {
+
+xmpError_t XMPAPI xmpIntegersAddAsync(xmpHandle_t handle, xmpIntegers_t s, const xmpIntegers_t a,
+                                      const xmpIntegers_t b, uint32_t count) {
+  return binary_async(Bin::Add, handle, s, a, b, count);
+}
+xmpError_t XMPAPI xmpIntegersAdd(xmpHandle_t handle, xmpIntegers_t s, const xmpIntegers_t a, const xmpIntegers_t b,
+                                 uint32_t count) {
+  return sync(handle, xmpIntegersAddAsync(handle, s, a, b, count));
+}
+xmpError_t XMPAPI xmpIntegersSubAsync(xmpHandle_t handle, xmpIntegers_t d, const xmpIntegers_t a,
+                                      const xmpIntegers_t b, uint32_t count) {
+  return binary_async(Bin::Sub, handle, d, a, b, count);
+}
+xmpError_t XMPAPI xmpIntegersSub(xmpHandle_t handle, xmpIntegers_t d, const xmpIntegers_t a, const xmpIntegers_t b,
+                                 uint32_t count) {
+  return sync(handle, xmpIntegersSubAsync(handle, d, a, b, count));
+}
+xmpError_t XMPAPI xmpIntegersSqrAsync(xmpHandle_t handle, xmpIntegers_t p, const xmpIntegers_t a, uint32_t count) {
+  xmpExecutionPolicy_t pol = handle->policy;
+  if (p->device != handle->device || a->device != handle->device) return xmpErrorInvalidDevice;
+  if (count == 0) return xmpErrorSuccess;
+  XMP_SET_DEVICE(handle);
+  if (p->count < count) return xmpErrorInvalidCount;
+  if (pol->indices[0] && pol->indices_count[0] < count) return xmpErrorInvalidCount;
+  if (p->precision < a->precision) return xmpErrorInvalidPrecision;
+  bool inplace = (p == a);
+  uint32_t *buffer = p->limbs;
+  if (inplace) {
+    xmpError_t e = set_necessary_scratch(handle, out_bytes(p));
+    if (e != xmpErrorSuccess) return e;
+    buffer = (uint32_t *)handle->scratch;
+    if (pol->indices[0] &&
+        hipMemcpyAsync(buffer, p->limbs, out_bytes(p), hipMemcpyDeviceToDevice, handle->stream) != hipSuccess)
+      return xmpErrorCuda;
+  }
+  Operand A = make_in(a, pol, 1);
+  if (launch_mul(make_out(p, buffer, pol, 0), A, A, count, handle->stream) != hipSuccess) return xmpErrorCuda;
+  return finish(handle, inplace, p, buffer);
+}
+xmpError_t XMPAPI xmpIntegersMulAsync(xmpHandle_t handle, xmpIntegers_t p, const xmpIntegers_t a,
+                                      const xmpIntegers_t b, uint32_t count) {
+  xmpExecutionPolicy_t pol = handle->policy;
+  // reference mul.cu:147-149: a == b without input indices is a square
+  if (a == b && !pol->indices[1] && !pol->indices[2]) return xmpIntegersSqrAsync(handle, p, a, count);
+  return binary_async(Bin::Mul, handle, p, a, b, count);
+}
+xmpError_t XMPAPI xmpIntegersMul(xmpHandle_t handle, xmpIntegers_t p, const xmpIntegers_t a, const xmpIntegers_t b,
+                                 uint32_t count) {
+  return sync(handle, xmpIntegersMulAsync(handle, p, a, b, count));
+}
+
+// ---- division ----------------------------------------------------------
+static xmpError_t divmod_async(xmpHandle_t h, xmpIntegers_t q, xmpIntegers_t m, xmpIntegers_t a, xmpIntegers_t b,
+                               uint32_t count, bool divmod) {
+  xmpExecutionPolicy_t p = h->policy;
+  if ((q && q->device != h->device) || (m && m->device != h->device) || a->device != h->device ||
+      b->device != h->device)
+    return xmpErrorInvalidDevice;
+  if (count == 0) return xmpErrorSuccess;
+  XMP_SET_DEVICE(h);
+  if ((q && q->count < count) || (m && m->count < count)) return xmpErrorInvalidCount;
+  if (p->indices[0] && p->indices_count[0] < count) return xmpErrorInvalidCount;
+  if (divmod && p->indices[1] && p->indices_count[1] < count) return xmpErrorInvalidCount;
+  if (q && q->precision < a->precision) return xmpErrorInvalidPrecision;
+  if (m && m->precision < b->precision) return xmpErrorInvalidPrecision;
+  // index slots: Div/Mod use 0 = out, 1 = a, 2 = b; DivMod 0 = q, 1 = r, 2 = a, 3 = b (div.cu:407-412)
+  const int sa = divmod ? 2 : 1, sb = divmod ? 3 : 2, sq = 0, sr = divmod ? 1 : 0;
+  bool qin = q && (q == a || q == b), min_ = m && (m == a || m == b);
+  size_t qbytes = qin ? out_bytes(q) : 0, mbytes = min_ ? out_bytes(m) : 0;
+  size_t ws = divmod_scratch_words(b->nlimbs, count) * sizeof(uint32_t);
+  xmpError_t e = set_necessary_scratch(h, qbytes + mbytes + ws);
+  if (e != xmpErrorSuccess) return e;
+  uint8_t *scr = (uint8_t *)h->scratch;
+  uint32_t *qbuf = q ? (qin ? (uint32_t *)scr : q->limbs) : nullptr;
+  uint32_t *mbuf = m ? (min_ ? (uint32_t *)(scr + qbytes) : m->limbs) : nullptr;
+  uint32_t *wsp = ws ? (uint32_t *)(scr + qbytes + mbytes) : nullptr;
+  if (qin && p->indices[sq] && hipMemcpyAsync(qbuf, q->limbs, qbytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+    return xmpErrorCuda;
+  if (min_ && p->indices[sr] &&
+      hipMemcpyAsync(mbuf, m->limbs, mbytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+    return xmpErrorCuda;
+  OutOperand Q = q ? make_out(q, qbuf, p, sq) : OutOperand{nullptr, 0, 0, nullptr};
+  OutOperand R = m ? make_out(m, mbuf, p, sr) : OutOperand{nullptr, 0, 0, nullptr};
+  if (launch_divmod_ws(Q, R, make_in(a, p, sa), make_in(b, p, sb), count, wsp, h->stream) != hipSuccess)
+    return xmpErrorCuda;
+  if (qin && hipMemcpyAsync(q->limbs, qbuf, qbytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+    return xmpErrorCuda;
+  if (min_ && hipMemcpyAsync(m->limbs, mbuf, mbytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+    return xmpErrorCuda;
+  XMP_CHECK_HIP();
+  return xmpErrorSuccess;
+}
+
+xmpError_t XMPAPI xmpIntegersDivAsync(xmpHandle_t handle, xmpIntegers_t q, const xmpIntegers_t a,
+                                      const xmpIntegers_t b, uint32_t count) {
+  return divmod_async(handle, q, nullptr, a, b, count, false);
+}
+xmpError_t XMPAPI xmpIntegersDiv(xmpHandle_t handle, xmpIntegers_t q, const xmpIntegers_t a, const xmpIntegers_t b,
+                                 uint32_t count) {
+  return sync(handle, xmpIntegersDivAsync(handle, q, a, b, count));
+}
+xmpError_t XMPAPI xmpIntegersModAsync(xmpHandle_t handle, xmpIntegers_t m, const xmpIntegers_t a,
+                                      const xmpIntegers_t b, uint32_t count) {
+  return divmod_async(handle, nullptr, m, a, b, count, false);
+}
+xmpError_t XMPAPI xmpIntegersMod(xmpHandle_t handle, xmpIntegers_t m, const xmpIntegers_t a, const xmpIntegers_t b,
+                                 uint32_t count) {
+  return sync(handle, xmpIntegersModAsync(handle, m, a, b, count));
+}
+xmpError_t XMPAPI xmpIntegersDivModAsync(xmpHandle_t handle, xmpIntegers_t q, xmpIntegers_t m,
+                                         const xmpIntegers_t a, const xmpIntegers_t b, uint32_t count) {
+  if (q == nullptr || m == nullptr) return xmpErrorInvalidParameter;
+  return divmod_async(handle, q, m, a, b, count, true);
+}
+xmpError_t XMPAPI xmpIntegersDivMod(xmpHandle_t handle, xmpIntegers_t q, xmpIntegers_t m, const xmpIntegers_t a,
+                                    const xmpIntegers_t b, uint32_t count) {
+  return sync(handle, xmpIntegersDivModAsync(handle, q, m, a, b, count));
+}
+
+// ---- powm --------------------------------------------------------------
+xmpError_t XMPAPI xmpIntegersPowmAsync(xmpHandle_t handle, xmpIntegers_t out, const xmpIntegers_t a,
+                                       const xmpIntegers_t exp, const xmpIntegers_t mod, uint32_t count) {
+  xmpExecutionPolicy_t p = handle->policy;
+  const int dev = handle->device;
+  if (out->device != dev || a->device != dev || exp->device != dev || mod->device != dev)
+    return xmpErrorInvalidDevice;
+  if (count == 0) return xmpErrorSuccess;
+  if (out->count < count) return xmpErrorInvalidCount;
+  if (p->indices[0] && p->indices_count[0] < count) return xmpErrorInvalidCount;
+  if (a->precision != out->precision || mod->precision != out->precision) return xmpErrorInvalidPrecision;
+  XMP_SET_DEVICE(handle);
+  PowmChoice c;
+  xmpError_t e = choose_powm(handle, out->precision, exp->precision, count, p->algorithm, &c);
+  if (e != xmpErrorSuccess) return e;
+  PowmVariant v = powm_variant(c.variant);
+
+  bool inplace = (out == a || out == exp || out == mod);
+  size_t obytes = inplace ? out_bytes(out) : 0;
+  uint32_t *obuf = out->limbs;
+  if (inplace) {
+    e = set_necessary_tmp_out(handle, obytes);
+    if (e != xmpErrorSuccess) return e;
+    obuf = (uint32_t *)handle->tmpOut;
+    if (p->indices[0] &&
+        hipMemcpyAsync(obuf, out->limbs, obytes, hipMemcpyDeviceToDevice, handle->stream) != hipSuccess)
+      return xmpErrorCuda;
+  }
+  // workspace: R^2 mod m per modulus, division workspace, window table per chunk
+  const uint32_t mstride = round_up(mod->count, kStrideAlign);
+  const size_t r2_words = (size_t)mod->nlimbs * mstride;
+  const size_t div_words = mod->nlimbs > kMaxRegDivisorLimbs ? (size_t)(2 * mod->nlimbs + 1) * round_up(mod->count, 256) : 0;
+  size_t limit = p->scratch_size_limit ? p->scratch_size_limit : handle->memorySize / 4 * 3;
+  size_t fixed = (r2_words + div_words) * sizeof(uint32_t);
+  uint32_t wbits = c.window_bits;
+  uint32_t chunk = count;
+  // shrink the chunk (and then the window) until the table fits the limit
+  // (reference powm_operators.h:308-328 shrinks the window only)
+  while (true) {
+    size_t need = fixed + powm_table_words(v, round_up(chunk, kStrideAlign), wbits) * sizeof(uint32_t);
+    if (need <= limit) break;
+    if (chunk > 65536) {
+      chunk = round_up(chunk / 2, kStrideAlign);
+    } else if (wbits > 2) {
+      wbits--;
+    } else {
+      return xmpErrorIncreaseScratchLimit;
+    }
+  }
+  size_t need = fixed + powm_table_words(v, round_up(chunk, kStrideAlign), wbits) * sizeof(uint32_t);
+  e = set_necessary_scratch(handle, need);
+  if (e != xmpErrorSuccess) return e;
+  uint32_t *r2 = (uint32_t *)handle->scratch;
+  uint32_t *divws = r2 + r2_words;
+  uint32_t *table = divws + div_words;
+  if (launch_pow2_mod(r2, mod->nlimbs, mstride, mod->limbs, mod->nlimbs, mod->stride, mod->count,
+                      2 * variant_rbits(v), div_words ? divws : nullptr, handle->stream) != hipSuccess)
+    return xmpErrorCuda;
+  PowmArgs args;
+  args.out = make_out(out, obuf, p, 0);
+  args.base = make_in(a, p, 1);
+  args.exp = make_in(exp, p, 2);
+  args.mod = make_in(mod, p, 3);
+  args.r2 = r2;
+  args.r2_stride = mstride;
+  args.table = table;
+  args.window_bits = wbits;
+  for (uint32_t start = 0; start < count; start += chunk) {
+    args.start = start;
+    args.count = std::min(chunk, count - start);
+    if (launch_powm_variant(c.variant, args, handle->stream) != hipSuccess) return xmpErrorCuda;
+  }
+  if (inplace && hipMemcpyAsync(out->limbs, obuf, obytes, hipMemcpyDeviceToDevice, handle->stream) != hipSuccess)
+    return xmpErrorCuda;
+  XMP_CHECK_HIP();
+  return xmpErrorSuccess;
+}
+
+xmpError_t XMPAPI xmpIntegersPowm(xmpHandle_t handle, xmpIntegers_t out, const xmpIntegers_t base,
+                                  const xmpIntegers_t exp, const xmpIntegers_t mod, uint32_t count) {
+  return sync(handle, xmpIntegersPowmAsync(handle, out, base, exp, mod, count));
+}
+
+// ---- bitwise operations: outside this library's scope (reference logic.cu) ----
+xmpError_t XMPAPI xmpIntegersShf(xmpHandle_t, xmpIntegers_t, const xmpIntegers_t, const int32_t *, const uint32_t,
+                                 uint32_t) {
+  return xmpErrorUnsupported;
+}
+xmpError_t XMPAPI xmpIntegersShfAsync(xmpHandle_t, xmpIntegers_t, const xmpIntegers_t, const int32_t *,
+                                      const uint32_t, uint32_t) {
+  return xmpErrorUnsupported;
+}
+#define XMP_UNSUPPORTED_BIN(name)                                                                       \
+  xmpError_t XMPAPI name(xmpHandle_t, xmpIntegers_t, const xmpIntegers_t, const xmpIntegers_t, uint32_t) { \
+    return xmpErrorUnsupported;                                                                         \
+  }
+XMP_UNSUPPORTED_BIN(xmpIntegersIor)
+XMP_UNSUPPORTED_BIN(xmpIntegersIorAsync)
+XMP_UNSUPPORTED_BIN(xmpIntegersAnd)
+XMP_UNSUPPORTED_BIN(xmpIntegersAndAsync)
+XMP_UNSUPPORTED_BIN(xmpIntegersXor)
+XMP_UNSUPPORTED_BIN(xmpIntegersXorAsync)
+#undef XMP_UNSUPPORTED_BIN
+xmpError_t XMPAPI xmpIntegersNot(xmpHandle_t, xmpIntegers_t, const xmpIntegers_t, uint32_t) {
+  return xmpErrorUnsupported;
+}
+xmpError_t XMPAPI xmpIntegersNotAsync(xmpHandle_t, xmpIntegers_t, const xmpIntegers_t, uint32_t) {
+  return xmpErrorUnsupported;
+}
+xmpError_t XMPAPI xmpIntegersPopc(xmpHandle_t, uint32_t *, const xmpIntegers_t, uint32_t) {
+  return xmpErrorUnsupported;
+}
+xmpError_t XMPAPI xmpIntegersPopcAsync(xmpHandle_t, uint32_t *, const xmpIntegers_t, uint32_t) {
+  return xmpErrorUnsupported;
+}
+xmpError_t XMPAPI xmpIntegersCmp(xmpHandle_t, int32_t *, const xmpIntegers_t, const xmpIntegers_t, uint32_t) {
+  return xmpErrorUnsupported;
+}
+xmpError_t XMPAPI xmpIntegersCmpAsync(xmpHandle_t, int32_t *, const xmpIntegers_t, const xmpIntegers_t, uint32_t) {
+  return xmpErrorUnsupported;
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+// powm.hpp -- Montgomery exponentiation plans and launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "internal.hpp"
+
+namespace xmp {
+
+constexpr uint32_t kMaxRegDivisorLimbs = 64;
+
+// division workspace (words) for divisors beyond the register kernels
+size_t divmod_scratch_words(uint32_t b_len, uint32_t count);
+hipError_t launch_divmod_ws(OutOperand q, OutOperand r, Operand a, Operand b, uint32_t count, uint32_t *scratch,
+                            hipStream_t stream);
+// out[:, t] = 2^e mod m[:, t] for t < count (interleaved, m_stride/out_stride)
+hipError_t launch_pow2_mod(uint32_t *out, uint32_t out_len, uint32_t out_stride, const uint32_t *m, uint32_t m_len,
+                           uint32_t m_stride, uint32_t count, uint32_t e, uint32_t *scratch, hipStream_t stream);
+
+// One Montgomery layout: an instance spans `lanes` lanes of a wave64, each
+// holding `digits` radix-2^radix digits (lanes*digits*radix >= precision+2).
+struct PowmVariant {
+  int lanes;    // L: 1, 2, 4, 8
+  int digits;   // K: digits per lane
+  int radix;    // D: 28 or 27
+};
+
+// Everything a powm launch needs (device pointers, interleaved layouts).
+struct PowmArgs {
+  OutOperand out;       // 32-bit limbs, out precision == mod precision
+  Operand base, exp, mod;
+  const uint32_t *r2;   // R^2 mod m per modulus (mod.len limbs, interleaved by modulus index)
+  uint32_t r2_stride;
+  uint32_t *table;      // window table workspace
+  uint32_t start;       // first instance of this launch
+  uint32_t count;       // instances in this launch
+  uint32_t window_bits;
+};
+
+// variants compiled into the library
+int powm_variant_count();
+PowmVariant powm_variant(int i);
+// index of the compiled variant (L, K, D) or -1
+int powm_find_variant(int lanes, int digits, int radix);
+// smallest R exponent (bits) a variant gives
+inline uint32_t variant_rbits(const PowmVariant &v) { return (uint32_t)(v.lanes * v.digits * v.radix); }
+// table workspace in 32-bit words for `count` instances
+size_t powm_table_words(const PowmVariant &v, uint32_t count, uint32_t window_bits);
+// launch the exponentiation kernel for one variant
+hipError_t launch_powm_variant(int variant, const PowmArgs &args, hipStream_t stream);
+// variants usable for a modulus precision (in bits): those with R >= 2^(precision+2)
+bool variant_fits(const PowmVariant &v, uint32_t precision);
+
+}  // namespace xmp

@@ -1,0 +1,163 @@
+// tuner.cpp -- runtime choice of powm layout: measured table, else model.
+#include "tuner.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace xmp {
+
+namespace {
+
+struct TunedEntry {
+  uint32_t precision, exp_bits, count;
+  int lanes, digits, radix;
+  uint32_t window;
+  double rate;
+};
+
+std::mutex g_mu;
+std::vector<TunedEntry> g_table;
+bool g_loaded = false;
+
+bool find_number(const std::string &obj, const char *key, double *out) {
+  std::string k = std::string("\"") + key + "\"";
+  size_t p = obj.find(k);
+  if (p == std::string::npos) return false;
+  p = obj.find(':', p);
+  if (p == std::string::npos) return false;
+  *out = std::strtod(obj.c_str() + p + 1, nullptr);
+  return true;
+}
+
+void parse_table(const std::string &text, std::vector<TunedEntry> *dst) {
+  size_t p = text.find("\"entries\"");
+  if (p == std::string::npos) return;
+  while (true) {
+    size_t a = text.find('{', p);
+    if (a == std::string::npos) break;
+    size_t b = text.find('}', a);
+    if (b == std::string::npos) break;
+    std::string obj = text.substr(a, b - a + 1);
+    double pr, eb, cnt, l, k, d, w, rate = 0;
+    if (find_number(obj, "precision", &pr) && find_number(obj, "exp_bits", &eb) && find_number(obj, "count", &cnt) &&
+        find_number(obj, "lanes", &l) && find_number(obj, "digits", &k) && find_number(obj, "radix", &d) &&
+        find_number(obj, "window", &w)) {
+      find_number(obj, "powm_per_s", &rate);
+      dst->push_back({(uint32_t)pr, (uint32_t)eb, (uint32_t)cnt, (int)l, (int)k, (int)d, (uint32_t)w, rate});
+    }
+    p = b + 1;
+  }
+}
+
+void ensure_loaded() {
+  if (g_loaded) return;
+  g_loaded = true;
+  const char *path = std::getenv("XMP_TUNE_FILE");
+  if (!path) return;
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return;
+  std::string text;
+  char buf[4096];
+  size_t n;
+  while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, n);
+  std::fclose(f);
+  parse_table(text, &g_table);
+}
+
+}  // namespace
+
+// Load (or replace) the measured table from JSON text.
+int tuner_load_text(const char *text) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_loaded = true;
+  std::vector<TunedEntry> t;
+  parse_table(text ? std::string(text) : std::string(), &t);
+  g_table.swap(t);
+  return (int)g_table.size();
+}
+
+uint32_t best_window_bits(uint32_t exp_bits) {
+  uint32_t best = 1;
+  double best_cost = 1e30;
+  for (uint32_t w = 1; w <= 8; w++) {
+    // table build (2^w products) + one multiply per window; squarings are fixed
+    double cost = (double)(1u << w) + std::ceil((double)exp_bits / w);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = w;
+    }
+  }
+  return best;
+}
+
+double model_cost(const PowmVariant &v, uint32_t precision, uint32_t exp_bits, uint32_t count, uint32_t window_bits,
+                  uint32_t cu_count) {
+  (void)precision;
+  const double nd = (double)(v.lanes * v.digits);
+  // per CIOS step: 2K multiply-adds plus the quotient/shift work (cross-lane when L > 1)
+  const double step = 2.0 * v.digits + (v.lanes > 1 ? 14.0 : 5.0);
+  const double mulmod = nd * step + 4.0 * v.digits * v.lanes;   // + normalisation
+  const double products = (double)(1u << window_bits) + exp_bits + std::ceil((double)exp_bits / window_bits) + 2;
+  const double per_wave = products * mulmod;
+  // waves in flight: two per SIMD (register budget), four SIMDs per CU
+  const double slots = 2.0 * 4.0 * cu_count;
+  const double waves = std::ceil((double)count * v.lanes / 64.0);
+  // a wave alone on its SIMD issues the 64-bit multiply at half rate
+  const double rounds = std::ceil(waves / slots);
+  const double fill = std::min(1.0, waves / (slots / 2.0));
+  return rounds * per_wave * (fill < 1.0 ? 2.0 : 1.0);
+}
+
+xmpError_t choose_powm(xmpHandle_t h, uint32_t precision, uint32_t exp_bits, uint32_t count, xmpAlgorithm_t alg,
+                       PowmChoice *out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  ensure_loaded();
+  auto allowed = [&](const PowmVariant &v) {
+    if (!variant_fits(v, precision)) return false;
+    if (alg == xmpAlgorithmRegMP) return v.lanes == 1;
+    if (alg == xmpAlgorithmDistributedMP) return v.lanes > 1;
+    return true;
+  };
+  // measured entries for this precision: nearest count (log scale), same exponent size preferred
+  const TunedEntry *best = nullptr;
+  double best_d = 1e30;
+  for (const TunedEntry &e : g_table) {
+    if (e.precision != precision) continue;
+    int vi = powm_find_variant(e.lanes, e.digits, e.radix);
+    if (vi < 0 || !allowed(powm_variant(vi))) continue;
+    double d = std::fabs(std::log((double)e.count + 1) - std::log((double)count + 1)) +
+               (e.exp_bits == exp_bits ? 0.0 : 10.0);
+    if (d < best_d) {
+      best_d = d;
+      best = &e;
+    }
+  }
+  if (best) {
+    out->variant = powm_find_variant(best->lanes, best->digits, best->radix);
+    out->window_bits = best->exp_bits == exp_bits ? best->window : best_window_bits(exp_bits);
+    return xmpErrorSuccess;
+  }
+  const uint32_t w = best_window_bits(exp_bits);
+  int bi = -1;
+  double bc = 1e300;
+  for (int i = 0; i < powm_variant_count(); i++) {
+    PowmVariant v = powm_variant(i);
+    if (!allowed(v)) continue;
+    double c = model_cost(v, precision, exp_bits, count, w, h->cuCount ? h->cuCount : 256);
+    if (c < bc) {
+      bc = c;
+      bi = i;
+    }
+  }
+  if (bi < 0) return xmpErrorUnsupported;
+  out->variant = bi;
+  out->window_bits = w;
+  return xmpErrorSuccess;
+}
+
+}  // namespace xmp
