@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Benchmark: 2048-bit modular exponentiation throughput on one MI355X.
+
+A step is one xmpIntegersPowm over 65,536 instances with 2048-bit odd moduli
+(top bit set, uniform random limbs), bases uniform below the modulus and
+uniform random 2048-bit exponents -- inputs resident in HBM.  The JSON line
+also carries the kernel's roofline, GMP's throughput on this host's cores,
+the elementwise add/sub/mul and div/mod rates, the mixed-size batched-vs-plain
+comparison and the serving loop's latency percentiles.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import xmp_amd  # noqa: E402
+from xmp_amd import hip, inputs  # noqa: E402
+from xmp_amd.ext import Montgomery  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# one v_mad_u64_u32 (32x32+64 -> 64) per lane per clock on 256 CUs x 64 lanes
+# at 2.4 GHz = the published 78.6 TFLOP/s FP64 vector rate of the same datapath;
+# a multiply-add counts as 2 integer operations
+VALU_PEAK_TOPS = 78.6
+
+
+def canonical_powm_macs(bits, exp_bits):
+    """(b + ceil(b/w)) * (2n^2 + n) 32x32 multiply-adds per powm, w from the
+    reference's windowBitsForPrecision (src/include/powm_operators.h:15-31)."""
+    n = bits // 32
+    w = 4 if exp_bits < 400 else 5 if exp_bits < 800 else 6 if exp_bits < 1600 else 7 if exp_bits <= 4096 else 8
+    return (exp_bits + math.ceil(exp_bits / w)) * (2 * n * n + n)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_async(fn, stream, reps):
+    """Average ms per call of fn (enqueued on `stream`), measured with HIP events."""
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.sync()
+    return e0.elapsed_ms(e1) / reps
+
+
+def cpu_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def powm_workload(h, n, bits, seed):
+    limbs = bits // 32
+    m = inputs.odd_moduli(seed, n, limbs)
+    b = inputs.below(seed + 1, m)
+    e = inputs.uniform(seed + 2, n, limbs)
+    return m, b, e, h.from_array(m), h.from_array(b), h.from_array(e)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the instances are independent, so N GPUs would each run this "
+              "same per-GPU workload on their own share; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from oracle import oracle as O   # checker + CPU baseline only
+
+    N, BITS = 65536, 2048
+    h = xmp_amd.Handle()
+    stream = hip.stream_create()
+    h.set_stream(stream)
+    m, b, e, M, B, E = powm_workload(h, N, BITS, 2048)
+    OUT = h.integers(BITS, N)
+
+    def step():
+        h.powm(OUT, B, E, M, N, asynchronous=True)
+
+    log("warmup %d steps" % args.warmup)
+    for _ in range(args.warmup):
+        step()
+    hip.stream_sync(stream)
+    hip.device_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    hip.stream_sync(stream)
+    hip.device_sync()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = N * args.steps / (t1 - t0)
+    log("headline: %.1f powm/s (%.2f ms/step)" % (value, ms_per_step))
+
+    # parity on a sample (not timed)
+    got = OUT.get()
+    sample = np.arange(0, N, 257)
+    want, _, _ = O.gmp_batch(O.GO_POWM, sample.size, 64, b[sample], e[sample], m[sample], threads=cpu_cores())
+    parity = bool(np.array_equal(got[sample], want))
+    log("parity on %d sampled instances: %s" % (sample.size, parity))
+
+    # the exponentiation kernel alone (constants resident): roofline
+    ctx = Montgomery(h, M, BITS, N)
+    lanes, digits, radix, wbits = ctx.layout()
+    ctx.powm_async(OUT, B, E, N)
+    hip.stream_sync(stream)
+    kern_ms = time_async(lambda: ctx.powm_async(OUT, B, E, N), stream, max(3, args.steps // 2))
+    macs = canonical_powm_macs(BITS, BITS) * N
+    achieved = 2.0 * macs / (kern_ms * 1e-3) / 1e12
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+                "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+                "kernel": "powm_kernel<%d,%d,%d> window %d" % (lanes, digits, radix, wbits),
+                "kernel_ms": round(kern_ms, 4)}
+    log("powm kernel %.3f ms -> %.2f TOP/s (%.1f%% of %.1f)" % (kern_ms, achieved, 100 * achieved / VALU_PEAK_TOPS,
+                                                            VALU_PEAK_TOPS))
+    ctx.close()
+
+    extra = {}
+    if not args.quick:
+        # ---- elementwise add/sub/mul on 1,048,576 pairs of 2048-bit integers
+        n2 = 1 << 20
+        a2, b2 = inputs.ordered_pair(4242, n2, 64)
+        A2, B2 = h.from_array(a2), h.from_array(b2)
+        S2, P2 = h.integers(2048, n2), h.integers(4096, n2)
+        lib = xmp_amd.lib()
+        add = lambda: lib.xmpIntegersAddAsync(h.h, S2.x, A2.x, B2.x, n2)  # noqa: E731
+        sub = lambda: lib.xmpIntegersSubAsync(h.h, S2.x, A2.x, B2.x, n2)  # noqa: E731
+        mul = lambda: lib.xmpIntegersMulAsync(h.h, P2.x, A2.x, B2.x, n2)  # noqa: E731
+        add()
+        hip.stream_sync(stream)
+        add_ms = time_async(add, stream, 20)
+        sub_ms = time_async(sub, stream, 20)
+        mul()
+        hip.stream_sync(stream)
+        mul_ms = time_async(mul, stream, 10)
+        extra["add_2048"] = {"pairs": n2, "ms": round(add_ms, 4), "GB_per_s": round(768 * n2 / add_ms / 1e6, 1),
+                             "hbm_frac": round(768 * n2 / add_ms / 1e6 / HBM_PEAK_GBS, 4)}
+        extra["sub_2048"] = {"pairs": n2, "ms": round(sub_ms, 4), "GB_per_s": round(768 * n2 / sub_ms / 1e6, 1),
+                             "hbm_frac": round(768 * n2 / sub_ms / 1e6 / HBM_PEAK_GBS, 4)}
+        extra["mul_2048x2048"] = {"pairs": n2, "ms": round(mul_ms, 4), "mul_per_s": round(n2 / mul_ms * 1e3, 1),
+                                  "GB_per_s": round(1024 * n2 / mul_ms / 1e6, 1),
+                                  "TOP_per_s": round(2 * 4096 * n2 / mul_ms / 1e9, 3)}
+        log("add %.3f ms  sub %.3f ms  mul %.3f ms" % (add_ms, sub_ms, mul_ms))
+        del A2, B2, S2, P2
+
+        # ---- div/mod: 65,536 x (4096-bit / 2048-bit, divisor top bit set)
+        a3 = inputs.uniform(4343, N, 128)
+        d3 = inputs.uniform(4344, N, 64)
+        d3[:, -1] |= np.uint32(0x80000000)
+        A3, D3 = h.from_array(a3), h.from_array(d3)
+        Q3, R3 = h.integers(4096, N), h.integers(2048, N)
+        div = lambda: lib.xmpIntegersDivAsync(h.h, Q3.x, A3.x, D3.x, N)  # noqa: E731
+        mod = lambda: lib.xmpIntegersModAsync(h.h, R3.x, A3.x, D3.x, N)  # noqa: E731
+        div()
+        hip.stream_sync(stream)
+        div_ms = time_async(div, stream, 10)
+        mod_ms = time_async(mod, stream, 10)
+        extra["div_4096_by_2048"] = {"count": N, "ms": round(div_ms, 4), "div_per_s": round(N / div_ms * 1e3, 1)}
+        extra["mod_4096_by_2048"] = {"count": N, "ms": round(mod_ms, 4), "mod_per_s": round(N / mod_ms * 1e3, 1)}
+        log("div %.3f ms  mod %.3f ms" % (div_ms, mod_ms))
+
+        # ---- 4,096 x 1024-bit powm (the reference's CPU-runnable case)
+        m1, b1, e1, M1, B1, E1 = powm_workload(h, 4096, 1024, 1024)
+        O1 = h.integers(1024, 4096)
+        p1 = lambda: h.powm(O1, B1, E1, M1, 4096, asynchronous=True)  # noqa: E731
+        p1()
+        hip.stream_sync(stream)
+        p1_ms = time_async(p1, stream, 5)
+        extra["powm_1024_x4096"] = {"ms": round(p1_ms, 4), "powm_per_s": round(4096 / p1_ms * 1e3, 1)}
+        log("powm 1024 x4096: %.3f ms" % p1_ms)
+
+    # ---- CPU baseline: GMP mpz_powm on this host's cores, bounded sample
+    cpu = None
+    if not args.no_cpu and O.gmp_available():
+        cores = cpu_cores()
+        S = 4096 if args.quick else 8192
+        _, _, secs = O.gmp_batch(O.GO_POWM, S, 64, b[:S], e[:S], m[:S], threads=cores)
+        cpu = {"value": round(S / secs, 1), "unit": "powm/s", "cores": cores, "kind": "port",
+               "sample": "GMP %s mpz_powm (the reference test suite's oracle, unit_tests/test.cu:1214) on the first "
+                         "%d of the same 2048-bit instances, %d pthreads, %.1f s wall" % (O.gmp_version(), S, cores,
+                                                                                      secs)}
+        log("cpu baseline: %.1f powm/s on %d cores" % (S / secs, cores))
+
+    line = {
+        "metric": "2048-bit powm/sec on 1 MI355X, batched vs plain launch; p50/p99 latency",
+        "value": round(value, 1),
+        "unit": "powm/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "powm 2048-bit: 65,536 instances, odd 2048-bit moduli (top bit set), bases below m, "
+                               "2048-bit exponents; one xmpIntegersPowm per step",
+                   "instances": N, "bits": BITS, "exp_bits": BITS, "parallelism": "none (1 GPU)"},
+        "parity": {"bit_exact_vs_gmp": parity, "sampled": int(sample.size)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+    h.close()
+
+
+if __name__ == "__main__":
+    main()

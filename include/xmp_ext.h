@@ -1,0 +1,77 @@
+/*
+ * xmp_ext.h -- extensions of the MI355X big-integer library beyond the
+ * reference API (include/xmp.h).  None of these has a counterpart in the
+ * reference; each is listed with the reference mechanism it builds on.
+ *
+ *  - Montgomery contexts: moduli kept resident on the device together with
+ *    R^2 mod m and the chosen layout, so repeated exponentiations skip the
+ *    per-call set-up (the reference recomputes a*R mod m by long division on
+ *    every call, src/include/powm_implementations.h:379-443).
+ *  - Mixed-size batches: one persistent, graph-captured launch over several
+ *    precision classes (the reference issues one xmpIntegersPowm per class,
+ *    src/powm.cu:250-307).
+ *  - Tuner: measured choice of lane layout and window per precision
+ *    (replaces src/tune/tune.cu and the compiled tables of src/powm.cu:117-236).
+ *  - Serving: a continuous-batching request loop over resident moduli.
+ */
+#pragma once
+
+#include "xmp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- Montgomery contexts ---------------- */
+typedef struct _xmpMontgomery_t *xmpMontgomery_t;
+
+/* Precompute R^2 mod m for the `mod` array (all of its instances) and pick
+ * the Montgomery layout for exponents of exp_precision bits and about
+ * count_hint instances per call.  `mod` must outlive the context and must not
+ * be modified while the context is used.  Moduli must be odd. */
+xmpError_t XMPAPI xmpMontgomeryCreate(xmpHandle_t handle, xmpMontgomery_t *ctx, xmpIntegers_t mod,
+                                      uint32_t exp_precision, uint32_t count_hint);
+xmpError_t XMPAPI xmpMontgomeryDestroy(xmpHandle_t handle, xmpMontgomery_t ctx);
+/* layout chosen for the context: lanes per instance, digits per lane, digit bits, window bits */
+xmpError_t XMPAPI xmpMontgomeryGetLayout(xmpHandle_t handle, xmpMontgomery_t ctx, uint32_t *lanes,
+                                         uint32_t *digits, uint32_t *radix, uint32_t *window_bits);
+
+/* out = base^exp mod m using a context: one kernel launch on the handle's
+ * stream.  Index semantics as xmpIntegersPowm (policy slots 0..3; slot 3
+ * selects the modulus inside the context's array). */
+xmpError_t XMPAPI xmpIntegersPowmMontgomeryAsync(xmpHandle_t handle, xmpIntegers_t out, const xmpIntegers_t base,
+                                                 const xmpIntegers_t exp, xmpMontgomery_t ctx, uint32_t count);
+
+/* ---------------- mixed-size batches ---------------- */
+typedef struct _xmpBatch_t *xmpBatch_t;
+
+/* A batch of `classes` precision classes; class c computes
+ * out[c][i] = base[c][i]^exp[c][i] mod mod[c][i] for i < counts[c].
+ * The arrays are bound at creation (they must outlive the batch); their
+ * contents may change between launches.  The launch is one persistent kernel
+ * whose workgroups pull tiles from a device queue ordered by cost, captured
+ * in a HIP graph on first launch. */
+xmpError_t XMPAPI xmpBatchCreate(xmpHandle_t handle, xmpBatch_t *batch, uint32_t classes, xmpIntegers_t *out,
+                                 xmpIntegers_t *base, xmpIntegers_t *exp, xmpIntegers_t *mod,
+                                 const uint32_t *counts);
+xmpError_t XMPAPI xmpBatchDestroy(xmpHandle_t handle, xmpBatch_t batch);
+/* recompute the per-modulus constants (after the moduli changed) */
+xmpError_t XMPAPI xmpBatchPrepare(xmpHandle_t handle, xmpBatch_t batch);
+/* one persistent launch (graph replay) on the handle's stream */
+xmpError_t XMPAPI xmpBatchLaunchAsync(xmpHandle_t handle, xmpBatch_t batch);
+/* the same work as separate plain launches, one per class (reference behaviour) */
+xmpError_t XMPAPI xmpBatchLaunchPlainAsync(xmpHandle_t handle, xmpBatch_t batch);
+
+/* ---------------- tuner ---------------- */
+/* Measure every compiled layout and window near the model's choice for
+ * powm at `precision` bits with `count` instances on this device; append the
+ * best entries as JSON text to `json` (capacity `json_bytes`).  Returns the
+ * number of layouts measured in *measured. */
+xmpError_t XMPAPI xmpTunerRun(xmpHandle_t handle, uint32_t precision, uint32_t exp_precision, uint32_t count,
+                              char *json, size_t json_bytes, uint32_t *measured);
+/* Load a tuning table (JSON text as written by xmpTunerRun or the tune tool). */
+xmpError_t XMPAPI xmpTunerLoad(const char *json);
+
+#ifdef __cplusplus
+}
+#endif

@@ -138,9 +138,9 @@ def test_powm_indices_and_resident_moduli(handle):
     handle.set_indices(3, im)
     try:
         handle.powm(Out, B, E, M)
-        got = Out.get()
     finally:
-        handle.clear_indices()
+        handle.clear_indices()   # export would scatter through slot 0 too (reference xmp.cu:586-682)
+    got = Out.get()
     bb = b
     ee = e[ie[np.arange(n) % 32] % 8]
     mm = m[im[np.arange(n) % 64] % nm]
