@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 
 import xmp_amd  # noqa: E402
 from xmp_amd import hip, inputs  # noqa: E402
-from xmp_amd.ext import Montgomery  # noqa: E402
+from xmp_amd.ext import Batch, Montgomery  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # one v_mad_u64_u32 (32x32+64 -> 64) per lane per clock on 256 CUs x 64 lanes
@@ -70,6 +70,52 @@ def powm_workload(h, n, bits, seed):
     b = inputs.below(seed + 1, m)
     e = inputs.uniform(seed + 2, n, limbs)
     return m, b, e, h.from_array(m), h.from_array(b), h.from_array(e)
+
+
+MIXED_SIZES = (512, 1024, 2048, 3072, 4096, 8192)
+MIXED_WEIGHTS = (0.10, 0.20, 0.35, 0.15, 0.15, 0.05)
+
+
+def mixed_counts(total, seed):
+    """Class sizes of a mixed batch: `total` draws over MIXED_SIZES with MIXED_WEIGHTS."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    draws = rng.choice(len(MIXED_SIZES), size=total, p=MIXED_WEIGHTS)
+    return [int((draws == k).sum()) for k in range(len(MIXED_SIZES))]
+
+
+def run_mixed(h, stream, total, seed, O, reps=2):
+    """Mixed-size batch: one persistent graph launch vs one plain launch per class."""
+    counts = mixed_counts(total, seed)
+    host, outs, bases, exps, mods = [], [], [], [], []
+    for k, (bits, n) in enumerate(zip(MIXED_SIZES, counts)):
+        m, b, e, M, B, E = powm_workload(h, n, bits, seed + 10 * k)
+        host.append((m, b, e))
+        outs.append(h.integers(bits, n))
+        bases.append(B)
+        exps.append(E)
+        mods.append(M)
+    batch = Batch(h, outs, bases, exps, mods, counts)
+    batch.launch_async()
+    hip.stream_sync(stream)
+    batched_ms = time_async(batch.launch_async, stream, reps)
+    # parity of the batched results on a sample of every class
+    ok = True
+    for k, bits in enumerate(MIXED_SIZES):
+        m, b, e = host[k]
+        idx = np.linspace(0, counts[k] - 1, num=min(8, counts[k])).astype(int)
+        want, _, _ = O.gmp_batch(O.GO_POWM, idx.size, bits // 32, b[idx], e[idx], m[idx], threads=cpu_cores())
+        ok &= bool(np.array_equal(outs[k].get()[idx], want))
+    batch.launch_plain_async()
+    hip.stream_sync(stream)
+    plain_ms = time_async(batch.launch_plain_async, stream, reps)
+    batch.close()
+    work = sum(c * canonical_powm_macs(bits, bits) for bits, c in zip(MIXED_SIZES, counts))
+    eq2048 = work / canonical_powm_macs(2048, 2048)   # 2048-bit-powm equivalents
+    return {"instances": total, "class_counts": dict(zip([str(x) for x in MIXED_SIZES], counts)),
+            "batched_ms": round(batched_ms, 3), "plain_ms": round(plain_ms, 3),
+            "batched_2048eq_powm_per_s": round(eq2048 / batched_ms * 1e3, 1),
+            "plain_2048eq_powm_per_s": round(eq2048 / plain_ms * 1e3, 1),
+            "batched_over_plain": round(plain_ms / batched_ms, 3), "parity_sampled": ok}
 
 
 def main():
@@ -188,6 +234,12 @@ def main():
         p1_ms = time_async(p1, stream, 5)
         extra["powm_1024_x4096"] = {"ms": round(p1_ms, 4), "powm_per_s": round(4096 / p1_ms * 1e3, 1)}
         log("powm 1024 x4096: %.3f ms" % p1_ms)
+
+        # ---- mixed-size batch: 262,144 instances, one persistent launch vs plain per-class launches
+        mixed = run_mixed(h, stream, 262144, 777, O)
+        extra["mixed_batch"] = mixed
+        log("mixed batch: batched %.1f ms, plain %.1f ms (x%.2f), parity %s" % (
+            mixed["batched_ms"], mixed["plain_ms"], mixed["batched_over_plain"], mixed["parity_sampled"]))
 
     # ---- CPU baseline: GMP mpz_powm on this host's cores, bounded sample
     cpu = None

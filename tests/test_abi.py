@@ -1,0 +1,99 @@
+"""CPU: the C-ABI library loads and exports every entry point the headers
+declare; host-only logic (tuner table parsing, work formulas) behaves.
+
+No compute call is made here (there is no GPU in this container).
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("xmp.h", "xmp_ext.h")]
+LIB = os.path.join(ROOT, "xmp_amd", "libxmp_amd.so")
+
+
+def declared_symbols():
+    names = []
+    for h in HEADERS:
+        text = open(h).read()
+        names += re.findall(r"XMPAPI\s+(xmp\w+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_headers_declare_reference_api():
+    names = declared_symbols()
+    # the reference's exported math/transfer/handle surface (src/include/xmp.h:119-226)
+    for n in ["xmpHandleCreate", "xmpHandleCreateWithMemoryFunctions", "xmpHandleDestroy", "xmpHandleSetStream",
+              "xmpHandleGetStream", "xmpHandleSetExecutionPolicy", "xmpHandleGetMemoryFunctions",
+              "xmpHandleGetDevice", "xmpExecutionPolicyCreate", "xmpExecutionPolicyDestroy",
+              "xmpExecutionPolicySetIndices", "xmpExecutionPolicySetIndicesAsync", "xmpExecutionPolicySetParameter",
+              "xmpExecutionPolicyGetParameter", "xmpIntegersCreate", "xmpIntegersDestroy", "xmpIntegersGetPrecision",
+              "xmpIntegersGetCount", "xmpIntegersImport", "xmpIntegersImportAsync", "xmpIntegersExport",
+              "xmpIntegersExportAsync", "xmpIntegersSet", "xmpIntegersSetAsync", "xmpIntegersAdd",
+              "xmpIntegersAddAsync", "xmpIntegersSub", "xmpIntegersSubAsync", "xmpIntegersMul",
+              "xmpIntegersMulAsync", "xmpIntegersDiv", "xmpIntegersDivAsync", "xmpIntegersMod",
+              "xmpIntegersModAsync", "xmpIntegersDivMod", "xmpIntegersDivModAsync", "xmpIntegersPowm",
+              "xmpIntegersPowmAsync"]:
+        assert n in names, n
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(LIB), "xmp_amd/libxmp_amd.so is not built (run __graft_entry__.build())"
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\sT\s+(\w+)", out))
+    missing = [n for n in declared_symbols() if n not in exported]
+    assert not missing, missing
+    # nothing else leaks: C++ internals stay hidden
+    leaked = [s for s in exported if not s.startswith("xmp") and not s.startswith("_fini")
+              and not s.startswith("_init")]
+    assert not [s for s in leaked if s.startswith("_ZN3xmp")], leaked
+
+
+def test_library_loads_and_resolves():
+    assert os.path.exists(LIB)
+    lib = ctypes.CDLL(LIB)
+    for n in declared_symbols():
+        assert getattr(lib, n) is not None
+
+
+def test_library_carries_gfx950_code():
+    out = subprocess.run(["readelf", "-S", LIB], capture_output=True, text=True, check=True).stdout
+    assert ".hip_fatbin" in out
+    blob = open(LIB, "rb").read()
+    assert b"gfx950" in blob
+
+
+def test_tuner_table_parser_host_only():
+    import xmp_amd
+    from xmp_amd import ext
+    text = ('{"device": "gfx950", "entries": [{"precision": 2048, "exp_bits": 2048, "count": 65536, "lanes": 2, '
+            '"digits": 37, "radix": 28, "window": 6, "powm_per_s": 1.0e6}]}')
+    xmp_amd.lib()
+    ext.tuner_load(text)
+    ext.tuner_load('{"entries": []}')
+
+
+def test_error_codes_match_reference_ordinals():
+    import xmp_amd
+    # src/include/xmp.h:40-54
+    assert xmp_amd.ERRORS.index("xmpErrorSuccess") == 0
+    assert xmp_amd.ERRORS.index("xmpErrorIncreaseScratchLimit") == 9
+    assert xmp_amd.ERRORS.index("xmpErrorCuda") == 12
+    text = open(HEADERS[0]).read()
+    enum = re.search(r"typedef enum \{(.*?)\} xmpError_t;", text, re.S).group(1)
+    assert [x.strip() for x in enum.split(",") if x.strip()] == xmp_amd.ERRORS
+
+
+def test_canonical_work_formula():
+    """Algorithmic work per powm used for the roofline: (b + ceil(b/w)) (2n^2 + n)."""
+    import bench
+    assert bench.canonical_powm_macs(1024, 1024) == 1195 * 2080        # 2.49e6
+    assert bench.canonical_powm_macs(2048, 2048) == 2341 * 8256        # 1.93e7
+
+
+def test_bench_refuses_several_gpus():
+    r = subprocess.run(["python3", os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
+    assert r.returncode == 2 and "one GPU" in r.stderr

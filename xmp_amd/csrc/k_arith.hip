@@ -9,11 +9,10 @@
 // registers (v_add_co_u32/v_addc_co_u32).
 //
 // mul (reference src/mul.cu:140-303, digitmp/mul.h:24-55): p = a*b mod
-// 2^P(p).  Operands are split into 28-bit digits held in registers and the
-// product is formed column by column (product scanning) in 64-bit
-// accumulators: each partial product is one v_mad_u64_u32 and carries are
-// resolved once per column, because up to 256 products of 56 bits fit in 64
-// bits.
+// 2^P(p).  The longer operand is split into 28-bit digits held in registers
+// and the product is formed row by row in 64-bit "lazy" accumulators: each
+// partial product is one v_mad_u64_u32 and carries are resolved once per
+// retired digit, because up to 256 products of 56 bits fit in 64 bits.
 #include "devmp.hpp"
 #include "kernels.hpp"
 
@@ -110,8 +109,6 @@ __device__ __forceinline__ void load_digits(uint32_t (&d)[ND], const Operand &o,
   l[NL] = 0;
 #pragma unroll
   for (int k = 0; k < ND; k++) {
-    constexpr int dummy = 0;
-    (void)dummy;
     const int bit = kDigit * k;
     const int w = bit >> 5, sh = bit & 31;
     uint32_t lo = w < NL + 1 ? l[w] : 0u;
@@ -120,44 +117,62 @@ __device__ __forceinline__ void load_digits(uint32_t (&d)[ND], const Operand &o,
   }
 }
 
-// Product scanning over ND_A x ND_B digits; output converted back to 32-bit
-// limbs and truncated to p.len limbs.
-template <int NLA, int NLB>
-__global__ void __launch_bounds__(256) mul_digits_kernel(OutOperand p, Operand a, Operand b, uint32_t count) {
+// Row scanning (operand scanning) over radix-2^28 digits: the multiplicand's
+// NDA digits and NDA lazy 64-bit accumulators stay in registers; the
+// multiplier is streamed from HBM one digit per row through a bit buffer, and
+// each row retires one finished product digit, so the accumulator window
+// never grows.  Output digits are repacked into 32-bit limbs on the fly and
+// truncated to p.len limbs.
+template <int NLA>
+__global__ void __launch_bounds__(64, (NLA >= 64 ? 1 : 2)) mul_rows_kernel(OutOperand p, Operand a, Operand b, uint32_t count) {
   constexpr int NDA = (32 * NLA + kDigit - 1) / kDigit;
-  constexpr int NDB = (32 * NLB + kDigit - 1) / kDigit;
-  constexpr int NDP = NDA + NDB;
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= count) return;
-  uint32_t ai = in_index(a, t), bi = in_index(b, t), pi = out_index(p, t);
-  uint32_t da[NDA], db[NDB];
+  const uint32_t ai = in_index(a, t), bi = in_index(b, t), pi = out_index(p, t);
+  uint32_t da[NDA];
   load_digits<NLA, NDA>(da, a, ai);
-  load_digits<NLB, NDB>(db, b, bi);
-  uint64_t acc = 0;
-  // bit buffer for digits -> limbs repacking
-  uint64_t bits = 0;
-  int nbits = 0;
-  uint32_t limb = 0;
+  uint64_t acc[NDA];
 #pragma unroll
-  for (int k = 0; k < NDP; k++) {
-#pragma unroll
-    for (int i = 0; i < NDA; i++) {
-      const int j = k - i;
-      if (j >= 0 && j < NDB) acc += (uint64_t)da[i] * db[j];
-    }
-    uint32_t digit = (uint32_t)acc & kDigitMask;
-    acc >>= kDigit;
-    bits |= (uint64_t)digit << nbits;
-    nbits += kDigit;
-    if (nbits >= 32) {
-      if (limb < p.len) p.data[(size_t)limb * p.stride + pi] = (uint32_t)bits;
+  for (int k = 0; k < NDA; k++) acc[k] = 0;
+  uint64_t ibuf = 0;   // multiplier bits not yet consumed
+  uint32_t inb = 0, next = 0;
+  uint64_t obuf = 0;   // product bits not yet stored
+  uint32_t onb = 0, limb = 0;
+  const uint32_t ndb = (32u * b.len + kDigit - 1) / kDigit;
+  auto emit = [&](uint32_t digit) {
+    obuf |= (uint64_t)digit << onb;
+    onb += kDigit;
+    if (onb >= 32) {
+      if (limb < p.len) p.data[(size_t)limb * p.stride + pi] = (uint32_t)obuf;
       limb++;
-      bits >>= 32;
-      nbits -= 32;
+      obuf >>= 32;
+      onb -= 32;
     }
+  };
+  for (uint32_t i = 0; i < ndb; i++) {
+    if (inb < (uint32_t)kDigit) {
+      ibuf |= (uint64_t)ld_limb(b, bi, next++) << inb;
+      inb += 32;
+    }
+    const uint32_t d = (uint32_t)ibuf & kDigitMask;
+    ibuf >>= kDigit;
+    inb -= kDigit;
+    uint64_t f = acc[0] + (uint64_t)da[0] * d;
+#pragma unroll
+    for (int j = 1; j < NDA; j++) acc[j - 1] = acc[j] + (uint64_t)da[j] * d;
+    acc[NDA - 1] = 0;
+    acc[0] += f >> kDigit;
+    emit((uint32_t)f & kDigitMask);
   }
-  if (nbits > 0) {
-    if (limb < p.len) p.data[(size_t)limb * p.stride + pi] = (uint32_t)bits;
+  uint64_t c = 0;
+#pragma unroll
+  for (int k = 0; k < NDA; k++) {
+    uint64_t x = acc[k] + c;
+    emit((uint32_t)x & kDigitMask);
+    c = x >> kDigit;
+  }
+  if (onb > 0) {
+    if (limb < p.len) p.data[(size_t)limb * p.stride + pi] = (uint32_t)obuf;
     limb++;
   }
   for (uint32_t k = limb; k < p.len; k++) p.data[(size_t)k * p.stride + pi] = 0u;
@@ -193,10 +208,9 @@ __global__ void __launch_bounds__(256) mul_general_kernel(OutOperand p, Operand 
   }
 }
 
-template <int NLA, int NLB>
-hipError_t launch_mul_digits(OutOperand p, Operand a, Operand b, uint32_t count, hipStream_t stream) {
-  hipLaunchKernelGGL((mul_digits_kernel<NLA, NLB>), dim3(div_round_up(count, 256)), dim3(256), 0, stream, p, a, b,
-                     count);
+template <int NLA>
+hipError_t launch_mul_rows(OutOperand p, Operand a, Operand b, uint32_t count, hipStream_t stream) {
+  hipLaunchKernelGGL((mul_rows_kernel<NLA>), dim3(div_round_up(count, 64)), dim3(64), 0, stream, p, a, b, count);
   return hipGetLastError();
 }
 
@@ -227,17 +241,13 @@ hipError_t launch_sub(OutOperand d, Operand a, Operand b, uint32_t count, hipStr
 }
 
 hipError_t launch_mul(OutOperand p, Operand a, Operand b, uint32_t count, hipStream_t stream) {
-  // register-resident digit kernels for the common square shapes
-  if (a.len == b.len) {
-    switch (a.len) {
-      case 4: return launch_mul_digits<4, 4>(p, a, b, count, stream);
-      case 8: return launch_mul_digits<8, 8>(p, a, b, count, stream);
-      case 16: return launch_mul_digits<16, 16>(p, a, b, count, stream);
-      case 32: return launch_mul_digits<32, 32>(p, a, b, count, stream);
-      case 64: return launch_mul_digits<64, 64>(p, a, b, count, stream);
-      default: break;
-    }
-  }
+  // register-resident row kernels: the longer operand (a) in registers, any multiplier length
+  if (a.len <= 4) return launch_mul_rows<4>(p, a, b, count, stream);
+  if (a.len <= 8) return launch_mul_rows<8>(p, a, b, count, stream);
+  if (a.len <= 16) return launch_mul_rows<16>(p, a, b, count, stream);
+  if (a.len <= 32) return launch_mul_rows<32>(p, a, b, count, stream);
+  if (a.len <= 48) return launch_mul_rows<48>(p, a, b, count, stream);
+  if (a.len <= 64) return launch_mul_rows<64>(p, a, b, count, stream);
   hipLaunchKernelGGL(mul_general_kernel, dim3(div_round_up(count, 256)), dim3(256), 0, stream, p, a, b, count);
   return hipGetLastError();
 }

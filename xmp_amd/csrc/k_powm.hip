@@ -148,33 +148,34 @@ __device__ __forceinline__ uint32_t exp_bits(const Operand &e, uint32_t inst, ui
   return funnel(hi, lo, s) & ((1u << len) - 1);
 }
 
+// One tile: the 64/L instances [tile*64/L, tile*64/L + 64/L) of a launch,
+// processed by one wave.  `bbuf` is this wave's LDS staging area
+// (G * NDP words, 16-byte aligned).
 template <int L, int K, int D>
-__global__ void __launch_bounds__(64, 2) powm_kernel(PowmArgs args) {
+__device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, uint32_t *bbuf) {
   using M = Mont<L, K, D>;
   constexpr int ND = M::ND;
   constexpr int NDP = (ND + 3) / 4 * 4;
   constexpr int G = 64 / L;   // instances per wave
-  __shared__ __attribute__((aligned(16))) uint32_t bbuf[G * NDP];
 
   const int lane = threadIdx.x;
   const int grp = lane / L;
   M mont(lane);
   const int l = mont.g.l;
-  const uint32_t local = blockIdx.x * G + grp;
+  const uint32_t local = tile * G + grp;
   const bool active = local < args.count;
   const uint32_t li = active ? local : args.count - 1;   // clamp: inactive lanes recompute a valid instance
   const uint32_t gi = args.start + li;
   const uint32_t bi = in_index(args.base, gi), ei = in_index(args.exp, gi), mi = in_index(args.mod, gi);
   uint32_t *b = bbuf + grp * NDP;
 
-  // modulus, its R^2, the base, all as digits
-  uint32_t a[K], r2d[K];
+  // modulus as digits; R^2 mod m straight into the LDS multiplier buffer
+  uint32_t a[K];
 #pragma unroll
   for (int k = 0; k < K; k++) {
     uint32_t d = (uint32_t)(l * K + k);
     mont.m[k] = limb_digit<D>(args.mod.data, args.mod.stride, mi, args.mod.len, d);
-    r2d[k] = limb_digit<D>(args.r2, args.r2_stride, mi, args.mod.len, d);
-    a[k] = limb_digit<D>(args.base.data, args.base.stride, bi, args.base.len, d);
+    b[l * K + k] = limb_digit<D>(args.r2, args.r2_stride, mi, args.mod.len, d);
   }
   {
     // np0 = -m^-1 mod 2^D (Newton, as reference regmp/np0.h:24-33)
@@ -188,23 +189,19 @@ __global__ void __launch_bounds__(64, 2) powm_kernel(PowmArgs args) {
   const uint32_t wbits = args.window_bits;
   const uint32_t T = 1u << wbits;
   uint32_t *tab = args.table + (size_t)local * T * ND;   // table sized for whole waves
-  auto tab_at = [&](uint32_t e, int k) -> uint32_t * { return tab + ((size_t)e * K + k) * L + l; };
+#define tab_at(e, k) (tab + ((size_t)(e) * K + (k)) * L + l)
 
-  // slot 0 = R mod m = (R^2 mod m) * 1 / R
+  // slot 0 = R mod m = 1 * (R^2 mod m) / R
 #pragma unroll
-  for (int k = 0; k < K; k++) b[l * K + k] = (l == 0 && k == 0) ? 1u : 0u;
+  for (int k = 0; k < K; k++) a[k] = (l == 0 && k == 0) ? 1u : 0u;
   __syncthreads();
-  {
-    uint32_t one[K];
-    mont.mul(one, r2d, b);
+  mont.mul(a, a, b);
 #pragma unroll
-    for (int k = 0; k < K; k++) *tab_at(0, k) = one[k];
-  }
-  __syncthreads();
-  // slot 1 = base * R mod m:  b = R^2 mod m
+  for (int k = 0; k < K; k++) *tab_at(0, k) = a[k];
+  // slot 1 = base * R mod m
 #pragma unroll
-  for (int k = 0; k < K; k++) b[l * K + k] = r2d[k];
-  __syncthreads();
+  for (int k = 0; k < K; k++)
+    a[k] = limb_digit<D>(args.base.data, args.base.stride, bi, args.base.len, (uint32_t)(l * K + k));
   mont.mul(a, a, b);
   __syncthreads();
   // slots 2..T-1: slot[e] = slot[e-1] * slot[1]
@@ -243,6 +240,7 @@ __global__ void __launch_bounds__(64, 2) powm_kernel(PowmArgs args) {
     mont.mul(a, a, b);
     __syncthreads();
   }
+#undef tab_at
   // leave Montgomery form: a * 1 / R, which is <= m
 #pragma unroll
   for (int k = 0; k < K; k++) b[l * K + k] = (l == 0 && k == 0) ? 1u : 0u;
@@ -277,6 +275,17 @@ __global__ void __launch_bounds__(64, 2) powm_kernel(PowmArgs args) {
   }
 }
 
+template <int L, int K, int D>
+constexpr int lds_words() {
+  return (64 / L) * ((L * K + 3) / 4 * 4);
+}
+
+template <int L, int K, int D>
+__global__ void __launch_bounds__(64, 2) powm_kernel(PowmArgs args) {
+  __shared__ __attribute__((aligned(16))) uint32_t bbuf[lds_words<L, K, D>()];
+  powm_tile<L, K, D>(args, blockIdx.x, bbuf);
+}
+
 struct VariantEntry {
   PowmVariant v;
   void (*kernel)(PowmArgs);
@@ -299,9 +308,78 @@ const VariantEntry kVariants[] = {
     XMP_VARIANT(8, 19, 27),  // 4104 : 4096-bit on eight lanes
     XMP_VARIANT(8, 29, 27),  // 6264 : 6144-bit
     XMP_VARIANT(8, 38, 27),  // 8208 : 8192-bit
+    XMP_VARIANT(16, 19, 27), // 8208 : 8192-bit on sixteen lanes
 };
 #undef XMP_VARIANT
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+// ---- persistent mixed-size kernel --------------------------------------
+// Every wave pulls (class, tile) pairs from a device queue ordered by
+// decreasing cost and runs the tile with its class's layout, so tiles of all
+// precision classes share one launch and no SIMD idles at a class boundary.
+// Each variant's tile is its own (non-inlined) function: it gets its own
+// register allocation, and the call costs nothing next to a tile's runtime.
+template <int I>
+__device__ __attribute__((noinline)) void run_variant(const PowmArgs &args, uint32_t tile, uint32_t *lds);
+
+#define XMP_RUN(I, L, K, D)                                                                          \
+  template <>                                                                                        \
+  __device__ __attribute__((noinline)) void run_variant<I>(const PowmArgs &args, uint32_t tile, uint32_t *lds) { \
+    powm_tile<L, K, D>(args, tile, lds);                                                             \
+  }
+XMP_RUN(0, 1, 12, 28)
+XMP_RUN(1, 1, 20, 28)
+XMP_RUN(2, 1, 28, 28)
+XMP_RUN(3, 1, 37, 28)
+XMP_RUN(4, 2, 19, 28)
+XMP_RUN(5, 2, 28, 28)
+XMP_RUN(6, 2, 37, 28)
+XMP_RUN(7, 4, 19, 28)
+XMP_RUN(8, 4, 28, 28)
+XMP_RUN(9, 2, 55, 28)
+XMP_RUN(10, 4, 38, 27)
+XMP_RUN(11, 8, 19, 27)
+XMP_RUN(12, 8, 29, 27)
+XMP_RUN(13, 8, 38, 27)
+XMP_RUN(14, 16, 19, 27)
+#undef XMP_RUN
+static_assert(kNumVariants == 15, "run_variant<> must list every compiled variant in kVariants order");
+static_assert(9 < kNumVariants, "variant 9 is the unbatched 2x55 layout");
+
+__global__ void __launch_bounds__(64, 2) powm_batch_kernel(BatchArgs ba) {
+  extern __shared__ __attribute__((aligned(16))) uint32_t dyn_lds[];
+  while (true) {
+    uint32_t idx = 0;
+    if (threadIdx.x == 0) idx = atomicAdd(ba.counter, 1u);
+    idx = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)__shfl((int)idx, 0));
+    if (idx >= ba.ntiles) break;
+    const uint32_t cls = (uint32_t)__builtin_amdgcn_readfirstlane((int)ba.tile_class[idx]);
+    const uint32_t tile = (uint32_t)__builtin_amdgcn_readfirstlane((int)ba.tile_index[idx]);
+    const PowmArgs &args = ba.classes[cls];
+    switch (__builtin_amdgcn_readfirstlane(ba.class_variant[cls])) {
+      case 0: run_variant<0>(args, tile, dyn_lds); break;
+      case 1: run_variant<1>(args, tile, dyn_lds); break;
+      case 2: run_variant<2>(args, tile, dyn_lds); break;
+      case 3: run_variant<3>(args, tile, dyn_lds); break;
+      case 4: run_variant<4>(args, tile, dyn_lds); break;
+      case 5: run_variant<5>(args, tile, dyn_lds); break;
+      case 6: run_variant<6>(args, tile, dyn_lds); break;
+      case 7: run_variant<7>(args, tile, dyn_lds); break;
+      case 8: run_variant<8>(args, tile, dyn_lds); break;
+      case 10: run_variant<10>(args, tile, dyn_lds); break;
+      case 11: run_variant<11>(args, tile, dyn_lds); break;
+      case 12: run_variant<12>(args, tile, dyn_lds); break;
+      case 13: run_variant<13>(args, tile, dyn_lds); break;
+      case 14: run_variant<14>(args, tile, dyn_lds); break;
+      default: break;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void queue_reset_kernel(uint32_t *counter) {
+  if (threadIdx.x == 0) counter[threadIdx.x] = 0u;
+}
 
 }  // namespace
 
@@ -323,6 +401,20 @@ bool variant_fits(const PowmVariant &v, uint32_t precision) {
 
 size_t powm_table_words(const PowmVariant &v, uint32_t count, uint32_t window_bits) {
   return (size_t)count * ((size_t)1 << window_bits) * (size_t)(v.lanes * v.digits);
+}
+
+// the 2-lane x 55-digit layout spills as a called function: not batched
+bool powm_variant_batchable(int variant) { return variant >= 0 && variant < kNumVariants && variant != 9; }
+
+size_t powm_variant_lds_bytes(int variant) {
+  const PowmVariant &v = kVariants[variant].v;
+  return sizeof(uint32_t) * (size_t)(64 / v.lanes) * (size_t)((v.lanes * v.digits + 3) / 4 * 4);
+}
+
+hipError_t launch_powm_batch(const BatchArgs &ba, uint32_t grid, size_t lds_bytes, hipStream_t stream) {
+  hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, stream, ba.counter);
+  hipLaunchKernelGGL(powm_batch_kernel, dim3(grid), dim3(64), lds_bytes, stream, ba);
+  return hipGetLastError();
 }
 
 hipError_t launch_powm_variant(int variant, const PowmArgs &args, hipStream_t stream) {

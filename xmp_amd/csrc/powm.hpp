@@ -53,5 +53,21 @@ size_t powm_table_words(const PowmVariant &v, uint32_t count, uint32_t window_bi
 hipError_t launch_powm_variant(int variant, const PowmArgs &args, hipStream_t stream);
 // variants usable for a modulus precision (in bits): those with R >= 2^(precision+2)
 bool variant_fits(const PowmVariant &v, uint32_t precision);
+// whether the persistent mixed-size kernel runs this variant
+bool powm_variant_batchable(int variant);
+// LDS staging bytes one wave of the variant needs
+size_t powm_variant_lds_bytes(int variant);
+
+// Persistent mixed-size launch: tiles (class, tile) in queue order.
+struct BatchArgs {
+  const PowmArgs *classes;        // per class (device array)
+  const int32_t *class_variant;   // per class variant index (device array)
+  const uint32_t *tile_class;     // per queue entry
+  const uint32_t *tile_index;     // per queue entry: tile within the class
+  uint32_t ntiles;
+  uint32_t *counter;              // queue head, reset by the launch
+};
+// queue reset + persistent kernel on `stream` (graph-capturable: no memset)
+hipError_t launch_powm_batch(const BatchArgs &ba, uint32_t grid, size_t lds_bytes, hipStream_t stream);
 
 }  // namespace xmp

@@ -114,11 +114,12 @@ double model_cost(const PowmVariant &v, uint32_t precision, uint32_t exp_bits, u
 }
 
 xmpError_t choose_powm(xmpHandle_t h, uint32_t precision, uint32_t exp_bits, uint32_t count, xmpAlgorithm_t alg,
-                       PowmChoice *out) {
+                       PowmChoice *out, bool batch_only) {
   std::lock_guard<std::mutex> lk(g_mu);
   ensure_loaded();
   auto allowed = [&](const PowmVariant &v) {
     if (!variant_fits(v, precision)) return false;
+    if (batch_only && !powm_variant_batchable(powm_find_variant(v.lanes, v.digits, v.radix))) return false;
     if (alg == xmpAlgorithmRegMP) return v.lanes == 1;
     if (alg == xmpAlgorithmDistributedMP) return v.lanes > 1;
     return true;

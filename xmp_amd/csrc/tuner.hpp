@@ -27,7 +27,8 @@ uint32_t best_window_bits(uint32_t exp_bits);
 double model_cost(const PowmVariant &v, uint32_t precision, uint32_t exp_bits, uint32_t count, uint32_t window_bits,
                   uint32_t cu_count);
 
+// batch_only: restrict to layouts the persistent mixed-size kernel runs
 xmpError_t choose_powm(xmpHandle_t h, uint32_t precision, uint32_t exp_bits, uint32_t count, xmpAlgorithm_t alg,
-                       PowmChoice *out);
+                       PowmChoice *out, bool batch_only = false);
 
 }  // namespace xmp
