@@ -83,7 +83,7 @@ def mixed_counts(total, seed):
     return [int((draws == k).sum()) for k in range(len(MIXED_SIZES))]
 
 
-def run_mixed(h, stream, total, seed, O, reps=2):
+def run_mixed(h, stream, total, seed, reps=2):
     """Mixed-size batch: one persistent graph launch vs one plain launch per class."""
     counts = mixed_counts(total, seed)
     host, outs, bases, exps, mods = [], [], [], [], []
@@ -98,13 +98,12 @@ def run_mixed(h, stream, total, seed, O, reps=2):
     batch.launch_async()
     hip.stream_sync(stream)
     batched_ms = time_async(batch.launch_async, stream, reps)
-    # parity of the batched results on a sample of every class
-    ok = True
+    # a sample of every class's batched results, checked in the CPU leg
+    samples = []
     for k, bits in enumerate(MIXED_SIZES):
         m, b, e = host[k]
         idx = np.linspace(0, counts[k] - 1, num=min(8, counts[k])).astype(int)
-        want, _, _ = O.gmp_batch(O.GO_POWM, idx.size, bits // 32, b[idx], e[idx], m[idx], threads=cpu_cores())
-        ok &= bool(np.array_equal(outs[k].get()[idx], want))
+        samples.append((bits, b[idx], e[idx], m[idx], outs[k].get()[idx]))
     batch.launch_plain_async()
     hip.stream_sync(stream)
     plain_ms = time_async(batch.launch_plain_async, stream, reps)
@@ -115,7 +114,39 @@ def run_mixed(h, stream, total, seed, O, reps=2):
             "batched_ms": round(batched_ms, 3), "plain_ms": round(plain_ms, 3),
             "batched_2048eq_powm_per_s": round(eq2048 / batched_ms * 1e3, 1),
             "plain_2048eq_powm_per_s": round(eq2048 / plain_ms * 1e3, 1),
-            "batched_over_plain": round(plain_ms / batched_ms, 3), "parity_sampled": ok}
+            "batched_over_plain": round(plain_ms / batched_ms, 3)}, samples
+
+
+def run_serving(h, nkeys, requests, seed):
+    """Serving loop over `nkeys` resident 2048-bit keys: saturation throughput,
+    then Poisson arrivals at 50% and 80% of it; p50/p99 latency per rate."""
+    from xmp_amd.ext import Server, make_keys
+    keys = make_keys(nkeys, 2048, seed)
+    rng = np.random.default_rng(seed)
+    kinds = (rng.random(requests) < 0.5).astype(np.uint32)
+    key_idx = rng.integers(0, nkeys, requests).astype(np.uint32)
+    bases = inputs.uniform(seed + 1, requests, 64).copy()
+    nmat = keys["n"][key_idx]
+    bases[:, -1] = (bases[:, -1].astype(np.uint64) % nmat[:, -1].astype(np.uint64)).astype(np.uint32)
+    exps = inputs.uniform(seed + 2, requests, 64)
+    srv = Server(h, keys, 2048, streams=4, max_batch=16384)
+    out = {"keys": nkeys, "requests": requests, "mix": "50% 2048-bit powm, 50% CRT pairs of 1024-bit moduli",
+           "streams": 4, "max_batch": 16384}
+    try:
+        srv.run_poisson(1e9, kinds[:4096], key_idx[:4096], bases[:4096], exps[:4096], seed=seed)   # warm-up
+        res, lat, wall = srv.run_poisson(1e9, kinds, key_idx, bases, exps, seed=seed)
+        sat = requests / wall
+        out["saturation_req_per_s"] = round(sat, 1)
+        for frac in (0.5, 0.8):
+            res, lat, wall = srv.run_poisson(frac * sat, kinds, key_idx, bases, exps, seed=seed + 1)
+            out["rate_%d%%" % int(frac * 100)] = {
+                "offered_req_per_s": round(frac * sat, 1), "achieved_req_per_s": round(requests / wall, 1),
+                "p50_ms": round(float(np.percentile(lat, 50)) / 1e3, 3),
+                "p99_ms": round(float(np.percentile(lat, 99)) / 1e3, 3)}
+    finally:
+        srv.close()
+    sample = np.linspace(0, requests - 1, num=64).astype(int)
+    return out, (keys, kinds[sample], key_idx[sample], bases[sample], exps[sample], res[sample])
 
 
 def main():
@@ -130,8 +161,6 @@ def main():
         print("bench.py measures one GPU only: the instances are independent, so N GPUs would each run this "
               "same per-GPU workload on their own share; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-
-    from oracle import oracle as O   # checker + CPU baseline only
 
     N, BITS = 65536, 2048
     h = xmp_amd.Handle()
@@ -158,12 +187,7 @@ def main():
     value = N * args.steps / (t1 - t0)
     log("headline: %.1f powm/s (%.2f ms/step)" % (value, ms_per_step))
 
-    # parity on a sample (not timed)
-    got = OUT.get()
-    sample = np.arange(0, N, 257)
-    want, _, _ = O.gmp_batch(O.GO_POWM, sample.size, 64, b[sample], e[sample], m[sample], threads=cpu_cores())
-    parity = bool(np.array_equal(got[sample], want))
-    log("parity on %d sampled instances: %s" % (sample.size, parity))
+    got = OUT.get()   # checked against GMP in the CPU leg
 
     # the exponentiation kernel alone (constants resident): roofline
     ctx = Montgomery(h, M, BITS, N)
@@ -236,22 +260,41 @@ def main():
         log("powm 1024 x4096: %.3f ms" % p1_ms)
 
         # ---- mixed-size batch: 262,144 instances, one persistent launch vs plain per-class launches
-        mixed = run_mixed(h, stream, 262144, 777, O)
+        mixed, mixed_samples = run_mixed(h, stream, 262144, 777)
         extra["mixed_batch"] = mixed
-        log("mixed batch: batched %.1f ms, plain %.1f ms (x%.2f), parity %s" % (
-            mixed["batched_ms"], mixed["plain_ms"], mixed["batched_over_plain"], mixed["parity_sampled"]))
+        log("mixed batch: batched %.1f ms, plain %.1f ms (x%.2f)" % (
+            mixed["batched_ms"], mixed["plain_ms"], mixed["batched_over_plain"]))
+        # ---- serving: 1024 resident keys, Poisson arrivals
+        serving, serve_sample = run_serving(h, 1024, 40000, 99)
+        extra["serving"] = serving
+        log("serving: %s" % json.dumps(serving))
 
-    # ---- CPU baseline: GMP mpz_powm on this host's cores, bounded sample
-    cpu = None
-    if not args.no_cpu and O.gmp_available():
-        cores = cpu_cores()
-        S = 4096 if args.quick else 8192
-        _, _, secs = O.gmp_batch(O.GO_POWM, S, 64, b[:S], e[:S], m[:S], threads=cores)
-        cpu = {"value": round(S / secs, 1), "unit": "powm/s", "cores": cores, "kind": "port",
-               "sample": "GMP %s mpz_powm (the reference test suite's oracle, unit_tests/test.cu:1214) on the first "
-                         "%d of the same 2048-bit instances, %d pthreads, %.1f s wall" % (O.gmp_version(), S, cores,
-                                                                                      secs)}
-        log("cpu baseline: %.1f powm/s on %d cores" % (S / secs, cores))
+    # ---- CPU leg: GMP (the reference test suite's oracle) on this host's cores.
+    # It times mpz_powm on a bounded sample of the same instances (the CPU
+    # baseline) and, as the checker, compares the GPU results it has sampled.
+    cpu, parity = None, {}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        if O.gmp_available():
+            cores = cpu_cores()
+            S = 4096 if args.quick else 8192
+            want, _, secs = O.gmp_batch(O.GO_POWM, S, 64, b[:S], e[:S], m[:S], threads=cores)
+            cpu = {"value": round(S / secs, 1), "unit": "powm/s", "cores": cores, "kind": "port",
+                   "sample": "GMP %s mpz_powm (the reference test suite's oracle, unit_tests/test.cu:1214) on the "
+                             "first %d of the same 2048-bit instances, %d pthreads, %.1f s wall" % (
+                                 O.gmp_version(), S, cores, secs)}
+            log("cpu baseline: %.1f powm/s on %d cores" % (S / secs, cores))
+            parity["headline"] = bool(np.array_equal(got[:S], want))
+            if not args.quick:
+                ok = True
+                for bits, bb, ee, mm, gg in mixed_samples:
+                    w, _, _ = O.gmp_batch(O.GO_POWM, bb.shape[0], bits // 32, bb, ee, mm, threads=cores)
+                    ok &= bool(np.array_equal(gg, w))
+                parity["mixed_batch_sample"] = ok
+                keys, kk, ki, bb, ee, gg = serve_sample
+                parity["serving_sample"] = bool(np.array_equal(
+                    gg, O.serving_expected(keys, kk, ki, bb, ee, threads=cores)))
+            log("parity: %s" % parity)
 
     line = {
         "metric": "2048-bit powm/sec on 1 MI355X, batched vs plain launch; p50/p99 latency",
@@ -269,7 +312,7 @@ def main():
         "config": {"workload": "powm 2048-bit: 65,536 instances, odd 2048-bit moduli (top bit set), bases below m, "
                                "2048-bit exponents; one xmpIntegersPowm per step",
                    "instances": N, "bits": BITS, "exp_bits": BITS, "parallelism": "none (1 GPU)"},
-        "parity": {"bit_exact_vs_gmp": parity, "sampled": int(sample.size)},
+        "parity": {"bit_exact_vs_gmp": parity},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "extra": extra,

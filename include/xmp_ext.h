@@ -62,6 +62,41 @@ xmpError_t XMPAPI xmpBatchLaunchAsync(xmpHandle_t handle, xmpBatch_t batch);
 /* the same work as separate plain launches, one per class (reference behaviour) */
 xmpError_t XMPAPI xmpBatchLaunchPlainAsync(xmpHandle_t handle, xmpBatch_t batch);
 
+/* ---------------- serving ---------------- */
+typedef struct _xmpServer_t *xmpServer_t;
+
+typedef struct {
+  uint32_t streams;    /* batch slots in flight at once (each on its own stream) */
+  uint32_t max_batch;  /* requests per batch slot */
+  size_t hbm_budget;   /* device bytes for in-flight request state; 0 = streams * max_batch requests */
+} xmpServerConfig_t;
+
+/* Keys (host arrays, `nkeys` each, 32-bit little-endian limbs, compact): n
+ * (bits), p and q (bits/2, odd, coprime), dp and dq (bits/2), cp and cq
+ * (bits, with cp = 1 mod p, 0 mod q and cq = 0 mod p, 1 mod q).  They stay
+ * resident on the device with their Montgomery constants.  A scheduler
+ * thread batches arriving requests into free slots within the HBM budget. */
+xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const xmpServerConfig_t *config,
+                                  uint32_t bits, uint32_t nkeys, const uint32_t *n, const uint32_t *p,
+                                  const uint32_t *q, const uint32_t *dp, const uint32_t *dq, const uint32_t *cp,
+                                  const uint32_t *cq);
+xmpError_t XMPAPI xmpServerDestroy(xmpServer_t server);
+/* kind 0: base^exp mod n[key]; kind 1: (c^dp mod p)*cp + (c^dq mod q)*cq mod n
+ * for c = base (exp ignored).  base/exp: bits/32 words. */
+xmpError_t XMPAPI xmpServerSubmit(xmpServer_t server, uint32_t kind, uint32_t key, const uint32_t *base,
+                                  const uint32_t *exp, uint64_t tag);
+/* up to `max` completed requests: tags, results (bits/32 words each), latency in microseconds */
+xmpError_t XMPAPI xmpServerPoll(xmpServer_t server, uint32_t max, uint64_t *tags, uint32_t *results,
+                                double *latency_us, uint32_t *n_out);
+/* wait until every submitted request has completed */
+xmpError_t XMPAPI xmpServerDrain(xmpServer_t server);
+/* load generator: `requests` requests arriving as a Poisson process of the
+ * given rate (seeded); request i is (kinds[i], keys[i], bases[i], exps[i]).
+ * Fills results and per-request latency (indexed by request) and the wall time. */
+xmpError_t XMPAPI xmpServerRunPoisson(xmpServer_t server, double rate_per_s, uint32_t requests, uint64_t seed,
+                                      const uint32_t *kinds, const uint32_t *keys, const uint32_t *bases,
+                                      const uint32_t *exps, uint32_t *results, double *latency_us, double *wall_s);
+
 /* ---------------- tuner ---------------- */
 /* Measure every compiled layout and window near the model's choice for
  * powm at `precision` bits with `count` instances on this device; append the

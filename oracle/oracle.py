@@ -182,3 +182,23 @@ def from_int(v, n):
         out[i] = v & 0xFFFFFFFF
         v >>= 32
     return out
+
+
+def serving_expected(keys, kinds, key_idx, bases, exps, threads=8):
+    """Expected results of served requests: kind 0 is base^exp mod n (GMP);
+    kind 1 is the CRT pair of samples/crt_rsa/rsa.cc:205-219,
+    (c^dp mod p)*cp + (c^dq mod q)*cq mod n, with c reduced mod p and q first
+    (Python int)."""
+    n32 = keys["n"].shape[1]
+    out = np.zeros((len(kinds), n32), np.uint32)
+    a = np.nonzero(kinds == 0)[0]
+    if a.size:
+        out[a], _, _ = gmp_batch(GO_POWM, a.size, n32, bases[a], exps[a], keys["n"][key_idx[a]], threads=threads)
+    for i in np.nonzero(kinds == 1)[0]:
+        k = key_idx[i]
+        p, q, n = to_int(keys["p"][k]), to_int(keys["q"][k]), to_int(keys["n"][k])
+        c = to_int(bases[i])
+        mp = pow(c % p, to_int(keys["dp"][k]), p)
+        mq = pow(c % q, to_int(keys["dq"][k]), q)
+        out[i] = from_int((mp * to_int(keys["cp"][k]) + mq * to_int(keys["cq"][k])) % n, n32)
+    return out

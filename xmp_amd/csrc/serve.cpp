@@ -1,0 +1,514 @@
+// serve.cpp -- continuous-batching request loop over resident keys.
+//
+// No reference counterpart.  The nearest reference mechanisms are the Async
+// API on a user stream (src/include/xmp.h:125-138), policy index arrays that
+// gather a per-instance modulus from a resident array
+// (powm_implementations.h:417-418) and the CRT sequence of
+// samples/crt_rsa/rsa.cc:205-219 (c mod p, powm, c mod q, powm, mul, mul,
+// add, mod).
+//
+// Keys stay resident on the device with their Montgomery constants.  A
+// scheduler thread takes whatever requests have arrived, admits as many as
+// the HBM budget allows into a free batch slot, and launches that batch on
+// the slot's own stream; slots run concurrently, so requests arriving while
+// others are on the GPU are admitted as soon as a slot and memory free up and
+// their kernels overlap with the ones already running.  Requests are
+//   kind 0: out = base^exp mod n_k                         (2048-bit)
+//   kind 1: out = CRT(c^dp_k mod p_k, c^dq_k mod q_k) mod n_k (two 1024-bit
+//           exponentiations recombined on the device as (mp*cp + mq*cq) mod n).
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <random>
+#include <thread>
+#include <vector>
+
+#include "../../include/xmp_ext.h"
+#include "internal.hpp"
+#include "kernels.hpp"
+#include "powm.hpp"
+#include "tuner.hpp"
+
+namespace xmp {
+xmpError_t montgomery_prepare(xmpHandle_t h, xmpIntegers_t mod, const PowmVariant &v, uint32_t **r2_out,
+                              uint32_t *stride_out);
+}
+
+using namespace xmp;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// per-slot index staging: keyA[cap] keyPQ[2cap] cidx[2cap] lo[cap] hi[cap] keyB[cap]
+constexpr uint32_t kIdxWords = 8;
+
+struct Request {
+  uint32_t kind, key;
+  uint64_t tag;
+  std::vector<uint32_t> base, exp;   // n32 words each (exp unused for kind 1)
+  Clock::time_point arrival;
+};
+
+struct Done {
+  uint64_t tag;
+  std::vector<uint32_t> result;
+  double latency_us;
+};
+
+// device array in the interleaved layout
+struct DevArray {
+  uint32_t *data = nullptr;
+  uint32_t limbs = 0, stride = 0, count = 0;
+};
+
+}  // namespace
+
+struct _xmpServer_t {
+  xmpHandle_t h = nullptr;
+  xmpServerConfig_t cfg{};
+  uint32_t bits = 0, n32 = 0, h32 = 0, nkeys = 0;
+  // resident keys
+  xmpIntegers_t N = nullptr, PQ = nullptr, DPQ = nullptr, CPQ = nullptr;
+  uint32_t *n_r2 = nullptr, *pq_r2 = nullptr;
+  uint32_t n_r2_stride = 0, pq_r2_stride = 0;
+  int var_n = -1, var_pq = -1;
+  uint32_t w_n = 0, w_pq = 0;
+  size_t bytes_per_request = 0;
+  uint32_t max_inflight = 0;
+
+  struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    uint32_t cap = 0;
+    // pinned staging
+    uint32_t *h_in = nullptr;     // [A base | A exp | B c] compact, cap*n32 each
+    uint32_t *h_idx = nullptr;    // index arrays
+    uint32_t *h_out = nullptr;    // [A out | B out] compact
+    uint32_t *d_in = nullptr, *d_idx = nullptr, *d_out = nullptr;
+    DevArray a_base, a_exp, a_out, b_c, b_cm, b_pow, b_prod, b_sum, b_out;
+    uint32_t *table_a = nullptr, *table_b = nullptr;
+    std::vector<Request> reqs;
+    uint32_t na = 0, nb = 0;
+    bool busy = false;
+  };
+  std::vector<Slot> slots;
+
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Request> pending;
+  std::deque<Done> done;
+  uint64_t submitted = 0, completed = 0;
+  uint32_t inflight = 0;
+  std::atomic<bool> running{false};
+  std::thread worker;
+  xmpError_t worker_error = xmpErrorSuccess;
+};
+
+namespace {
+
+hipError_t alloc_array(xmpHandle_t h, DevArray &a, uint32_t limbs, uint32_t count) {
+  a.limbs = limbs;
+  a.count = count;
+  a.stride = round_up(count, kStrideAlign);
+  a.data = (uint32_t *)h->da(sizeof(uint32_t) * (size_t)limbs * a.stride);
+  return a.data ? hipSuccess : hipErrorOutOfMemory;
+}
+
+Operand op(const DevArray &a, const uint32_t *idx = nullptr, uint32_t idx_count = 0) {
+  return Operand{a.data, a.limbs, a.stride, a.count, idx, idx_count};
+}
+Operand op(xmpIntegers_t x, const uint32_t *idx = nullptr, uint32_t idx_count = 0) {
+  return Operand{x->limbs, x->nlimbs, x->stride, x->count, idx, idx_count};
+}
+OutOperand outop(const DevArray &a) { return OutOperand{a.data, a.limbs, a.stride, nullptr}; }
+
+// Launch one batch on its slot's stream: staging copies, the kernels for both
+// request kinds, result copy back and the completion event.
+hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
+  const uint32_t n32 = s->n32, h32 = s->h32;
+  const uint32_t na = sl.na, nb = sl.nb, cap = sl.cap;
+  hipStream_t st = sl.stream;
+  // index layout in h_idx: keyA[cap] | keyPQ[2cap] | cidx[2cap] | lo[cap] | hi[cap] | keyB[cap]
+  uint32_t *keyA = sl.h_idx, *keyPQ = keyA + cap, *cidx = keyPQ + 2 * cap, *lo = cidx + 2 * cap, *hi = lo + cap,
+           *keyB = hi + cap;
+  for (uint32_t i = 0; i < na; i++) keyA[i] = sl.reqs[i].key;
+  for (uint32_t j = 0; j < nb; j++) {
+    const Request &r = sl.reqs[na + j];
+    keyPQ[j] = r.key;
+    keyPQ[nb + j] = s->nkeys + r.key;
+    cidx[j] = j;
+    cidx[nb + j] = j;
+    lo[j] = j;
+    hi[j] = nb + j;
+    keyB[j] = r.key;
+  }
+  uint32_t *d_keyA = sl.d_idx, *d_keyPQ = d_keyA + cap, *d_cidx = d_keyPQ + 2 * cap, *d_lo = d_cidx + 2 * cap,
+           *d_hi = d_lo + cap, *d_keyB = d_hi + cap;
+  hipError_t e;
+  if ((e = hipMemcpyAsync(sl.d_idx, sl.h_idx, sizeof(uint32_t) * kIdxWords * cap, hipMemcpyHostToDevice, st)) != hipSuccess)
+    return e;
+  const size_t words = (size_t)cap * n32;
+  if ((e = hipMemcpyAsync(sl.d_in, sl.h_in, sizeof(uint32_t) * 3 * words, hipMemcpyHostToDevice, st)) != hipSuccess)
+    return e;
+  const uint8_t *in_base = (const uint8_t *)sl.d_in, *in_exp = (const uint8_t *)(sl.d_in + words),
+                *in_c = (const uint8_t *)(sl.d_in + 2 * words);
+  if (na) {
+    if ((e = launch_import(sl.a_base.data, sl.a_base.stride, n32, in_base, na, n32, -1, -1, 0, 4, nullptr, 0, st)) ||
+        (e = launch_import(sl.a_exp.data, sl.a_exp.stride, n32, in_exp, na, n32, -1, -1, 0, 4, nullptr, 0, st)))
+      return e;
+    PowmArgs a;
+    a.out = outop(sl.a_out);
+    a.base = op(sl.a_base);
+    a.exp = op(sl.a_exp);
+    a.mod = op(s->N, d_keyA, na);
+    a.r2 = s->n_r2;
+    a.r2_stride = s->n_r2_stride;
+    a.table = sl.table_a;
+    a.start = 0;
+    a.count = na;
+    a.window_bits = s->w_n;
+    if ((e = launch_powm_variant(s->var_n, a, st)) != hipSuccess) return e;
+    if ((e = launch_export((uint8_t *)sl.d_out, sl.a_out.data, sl.a_out.stride, n32, na, n32, -1, -1, 0, 4, nullptr,
+                           0, st)) != hipSuccess)
+      return e;
+  }
+  if (nb) {
+    if ((e = launch_import(sl.b_c.data, sl.b_c.stride, n32, in_c, nb, n32, -1, -1, 0, 4, nullptr, 0, st))) return e;
+    // c mod p_k and c mod q_k in one launch over 2*nb instances
+    OutOperand none{nullptr, 0, 0, nullptr};
+    if ((e = launch_divmod_ws(none, outop(sl.b_cm), op(sl.b_c, d_cidx, 2 * nb), op(s->PQ, d_keyPQ, 2 * nb), 2 * nb,
+                              nullptr, st)) != hipSuccess)
+      return e;
+    // mp = cm^dp mod p, mq = cm^dq mod q in one launch
+    PowmArgs a;
+    a.out = outop(sl.b_pow);
+    a.base = op(sl.b_cm);
+    a.exp = op(s->DPQ, d_keyPQ, 2 * nb);
+    a.mod = op(s->PQ, d_keyPQ, 2 * nb);
+    a.r2 = s->pq_r2;
+    a.r2_stride = s->pq_r2_stride;
+    a.table = sl.table_b;
+    a.start = 0;
+    a.count = 2 * nb;
+    a.window_bits = s->w_pq;
+    if ((e = launch_powm_variant(s->var_pq, a, st)) != hipSuccess) return e;
+    // sp = mp*cp, sq = mq*cq; r = (sp + sq) mod n
+    if ((e = launch_mul(outop(sl.b_prod), op(s->CPQ, d_keyPQ, 2 * nb), op(sl.b_pow), 2 * nb, st)) != hipSuccess)
+      return e;
+    if ((e = launch_add(outop(sl.b_sum), op(sl.b_prod, d_lo, nb), op(sl.b_prod, d_hi, nb), nb, st)) != hipSuccess)
+      return e;
+    if ((e = launch_divmod_ws(none, outop(sl.b_out), op(sl.b_sum), op(s->N, d_keyB, nb), nb, nullptr, st)) !=
+        hipSuccess)
+      return e;
+    if ((e = launch_export((uint8_t *)(sl.d_out + words), sl.b_out.data, sl.b_out.stride, n32, nb, n32, -1, -1, 0, 4,
+                           nullptr, 0, st)) != hipSuccess)
+      return e;
+  }
+  if ((e = hipMemcpyAsync(sl.h_out, sl.d_out, sizeof(uint32_t) * 2 * words, hipMemcpyDeviceToHost, st)) != hipSuccess)
+    return e;
+  (void)h32;
+  return hipEventRecord(sl.done, st);
+}
+
+void worker_loop(xmpServer_t s) {
+  (void)hipSetDevice(s->h->device);
+  const uint32_t n32 = s->n32;
+  while (true) {
+    bool progressed = false;
+    // 1. retire finished batches
+    for (auto &sl : s->slots) {
+      if (!sl.busy) continue;
+      hipError_t q = hipEventQuery(sl.done);
+      if (q == hipErrorNotReady) continue;
+      auto now = Clock::now();
+      std::vector<Done> out;
+      out.reserve(sl.reqs.size());
+      const size_t words = (size_t)sl.cap * n32;
+      for (uint32_t i = 0; i < sl.reqs.size(); i++) {
+        const Request &r = sl.reqs[i];
+        const uint32_t *src = (i < sl.na) ? sl.h_out + (size_t)i * n32 : sl.h_out + words + (size_t)(i - sl.na) * n32;
+        Done d;
+        d.tag = r.tag;
+        d.result.assign(src, src + n32);
+        d.latency_us = std::chrono::duration<double, std::micro>(now - r.arrival).count();
+        out.push_back(std::move(d));
+      }
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (q != hipSuccess) s->worker_error = xmpErrorCuda;
+        for (auto &d : out) s->done.push_back(std::move(d));
+        s->completed += sl.reqs.size();
+        s->inflight -= (uint32_t)sl.reqs.size();
+      }
+      s->cv.notify_all();
+      sl.reqs.clear();
+      sl.busy = false;
+      progressed = true;
+    }
+    // 2. admit pending requests into a free slot, within the HBM budget
+    for (auto &sl : s->slots) {
+      if (sl.busy) continue;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->pending.empty()) break;
+        uint32_t budget = s->max_inflight > s->inflight ? s->max_inflight - s->inflight : 0;
+        uint32_t take = std::min<uint32_t>({sl.cap, budget, (uint32_t)s->pending.size()});
+        if (take == 0) break;
+        // kind-0 requests first, then kind 1 (slot layout); order within a kind is arrival order
+        std::vector<Request> a, b;
+        for (uint32_t i = 0; i < take; i++) {
+          Request r = std::move(s->pending.front());
+          s->pending.pop_front();
+          (r.kind == 0 ? a : b).push_back(std::move(r));
+        }
+        sl.na = (uint32_t)a.size();
+        sl.nb = (uint32_t)b.size();
+        sl.reqs = std::move(a);
+        for (auto &r : b) sl.reqs.push_back(std::move(r));
+        s->inflight += take;
+      }
+      const size_t words = (size_t)sl.cap * n32;
+      for (uint32_t i = 0; i < sl.na; i++) {
+        std::memcpy(sl.h_in + (size_t)i * n32, sl.reqs[i].base.data(), n32 * 4);
+        std::memcpy(sl.h_in + words + (size_t)i * n32, sl.reqs[i].exp.data(), n32 * 4);
+      }
+      for (uint32_t j = 0; j < sl.nb; j++)
+        std::memcpy(sl.h_in + 2 * words + (size_t)j * n32, sl.reqs[sl.na + j].base.data(), n32 * 4);
+      if (launch_batch(s, sl) != hipSuccess) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->worker_error = xmpErrorCuda;
+      }
+      sl.busy = true;
+      progressed = true;
+    }
+    if (!s->running.load()) {
+      bool any_busy = false;
+      for (auto &sl : s->slots) any_busy |= sl.busy;
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (!any_busy && s->pending.empty()) break;
+    }
+    if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+}
+
+void free_server(xmpServer_t s) {
+  xmpHandle_t h = s->h;
+  for (auto &sl : s->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    for (DevArray *a : {&sl.a_base, &sl.a_exp, &sl.a_out, &sl.b_c, &sl.b_cm, &sl.b_pow, &sl.b_prod, &sl.b_sum,
+                        &sl.b_out})
+      if (a->data) h->df(a->data);
+    if (sl.table_a) h->df(sl.table_a);
+    if (sl.table_b) h->df(sl.table_b);
+    if (sl.d_in) h->df(sl.d_in);
+    if (sl.d_idx) h->df(sl.d_idx);
+    if (sl.d_out) h->df(sl.d_out);
+    if (sl.h_in) (void)hipHostFree(sl.h_in);
+    if (sl.h_idx) (void)hipHostFree(sl.h_idx);
+    if (sl.h_out) (void)hipHostFree(sl.h_out);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  }
+  for (xmpIntegers_t x : {s->N, s->PQ, s->DPQ, s->CPQ})
+    if (x) xmpIntegersDestroy(h, x);
+  if (s->n_r2) h->df(s->n_r2);
+  if (s->pq_r2) h->df(s->pq_r2);
+}
+
+}  // namespace
+
+extern "C" {
+
+xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const xmpServerConfig_t *config,
+                                  uint32_t bits, uint32_t nkeys, const uint32_t *n, const uint32_t *p,
+                                  const uint32_t *q, const uint32_t *dp, const uint32_t *dq, const uint32_t *cp,
+                                  const uint32_t *cq) {
+  XMP_CHECK_NE(server, nullptr);
+  XMP_CHECK_NE(config, nullptr);
+  if (!n || !p || !q || !dp || !dq || !cp || !cq) return xmpErrorInvalidParameter;
+  if (bits == 0 || bits % 64 != 0 || nkeys == 0) return xmpErrorInvalidParameter;
+  if (config->streams == 0 || config->max_batch == 0) return xmpErrorInvalidParameter;
+  XMP_SET_DEVICE(handle);
+  xmpServer_t s = new _xmpServer_t();
+  s->h = handle;
+  s->cfg = *config;
+  s->bits = bits;
+  s->n32 = bits / 32;
+  s->h32 = bits / 64;
+  s->nkeys = nkeys;
+  xmpError_t e = xmpErrorSuccess;
+  auto fail = [&](xmpError_t err) {
+    free_server(s);
+    delete s;
+    return err;
+  };
+  // resident keys: n (bits), [p;q] and [dp;dq] (bits/2), [cp;cq] (bits)
+  std::vector<uint32_t> pq(2 * (size_t)nkeys * s->h32), dpq(2 * (size_t)nkeys * s->h32),
+      cpq(2 * (size_t)nkeys * s->n32);
+  std::memcpy(pq.data(), p, sizeof(uint32_t) * nkeys * s->h32);
+  std::memcpy(pq.data() + (size_t)nkeys * s->h32, q, sizeof(uint32_t) * nkeys * s->h32);
+  std::memcpy(dpq.data(), dp, sizeof(uint32_t) * nkeys * s->h32);
+  std::memcpy(dpq.data() + (size_t)nkeys * s->h32, dq, sizeof(uint32_t) * nkeys * s->h32);
+  std::memcpy(cpq.data(), cp, sizeof(uint32_t) * nkeys * s->n32);
+  std::memcpy(cpq.data() + (size_t)nkeys * s->n32, cq, sizeof(uint32_t) * nkeys * s->n32);
+  if ((e = xmpIntegersCreate(handle, &s->N, bits, nkeys)) || (e = xmpIntegersCreate(handle, &s->PQ, bits / 2, 2 * nkeys)) ||
+      (e = xmpIntegersCreate(handle, &s->DPQ, bits / 2, 2 * nkeys)) ||
+      (e = xmpIntegersCreate(handle, &s->CPQ, bits, 2 * nkeys)) ||
+      (e = xmpIntegersImport(handle, s->N, s->n32, -1, 4, 0, 0, const_cast<uint32_t *>(n), nkeys)) ||
+      (e = xmpIntegersImport(handle, s->PQ, s->h32, -1, 4, 0, 0, pq.data(), 2 * nkeys)) ||
+      (e = xmpIntegersImport(handle, s->DPQ, s->h32, -1, 4, 0, 0, dpq.data(), 2 * nkeys)) ||
+      (e = xmpIntegersImport(handle, s->CPQ, s->n32, -1, 4, 0, 0, cpq.data(), 2 * nkeys)))
+    return fail(e);
+  const uint32_t cap = config->max_batch;
+  PowmChoice cn, cpqc;
+  if ((e = choose_powm(handle, bits, bits, cap, xmpAlgorithmDefault, &cn)) ||
+      (e = choose_powm(handle, bits / 2, bits / 2, 2 * cap, xmpAlgorithmDefault, &cpqc)))
+    return fail(e);
+  s->var_n = cn.variant;
+  s->w_n = cn.window_bits;
+  s->var_pq = cpqc.variant;
+  s->w_pq = cpqc.window_bits;
+  const PowmVariant vn = powm_variant(cn.variant), vpq = powm_variant(cpqc.variant);
+  if ((e = montgomery_prepare(handle, s->N, vn, &s->n_r2, &s->n_r2_stride)) ||
+      (e = montgomery_prepare(handle, s->PQ, vpq, &s->pq_r2, &s->pq_r2_stride)))
+    return fail(e);
+  // device bytes one in-flight request holds (kind-1 worst case): staging, intermediates, table share
+  const size_t tab_a = powm_table_words(vn, 1, cn.window_bits) * 4;
+  const size_t tab_b = powm_table_words(vpq, 2, cpqc.window_bits) * 4;
+  s->bytes_per_request = 4 * ((size_t)s->n32 * 8 + (size_t)s->h32 * 4 + 2 * ((size_t)s->n32 + s->h32) + s->n32 + 2) +
+                         std::max(tab_a, tab_b);
+  size_t budget = config->hbm_budget ? config->hbm_budget : s->bytes_per_request * cap * config->streams;
+  s->max_inflight = (uint32_t)std::max<size_t>(1, budget / s->bytes_per_request);
+  s->slots.resize(config->streams);
+  for (auto &sl : s->slots) {
+    sl.cap = cap;
+    const size_t words = (size_t)cap * s->n32;
+    if (hipStreamCreate(&sl.stream) != hipSuccess || hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc((void **)&sl.h_in, sizeof(uint32_t) * 3 * words, 0) != hipSuccess ||
+        hipHostMalloc((void **)&sl.h_idx, sizeof(uint32_t) * kIdxWords * cap, 0) != hipSuccess ||
+        hipHostMalloc((void **)&sl.h_out, sizeof(uint32_t) * 2 * words, 0) != hipSuccess)
+      return fail(xmpErrorCuda);
+    sl.d_in = (uint32_t *)handle->da(sizeof(uint32_t) * 3 * words);
+    sl.d_idx = (uint32_t *)handle->da(sizeof(uint32_t) * kIdxWords * cap);
+    sl.d_out = (uint32_t *)handle->da(sizeof(uint32_t) * 2 * words);
+    sl.table_a = (uint32_t *)handle->da(sizeof(uint32_t) * powm_table_words(vn, round_up(cap, kStrideAlign), cn.window_bits));
+    sl.table_b =
+        (uint32_t *)handle->da(sizeof(uint32_t) * powm_table_words(vpq, round_up(2 * cap, kStrideAlign), cpqc.window_bits));
+    if (!sl.d_in || !sl.d_idx || !sl.d_out || !sl.table_a || !sl.table_b) return fail(xmpErrorInvalidCudaMalloc);
+    if (alloc_array(handle, sl.a_base, s->n32, cap) || alloc_array(handle, sl.a_exp, s->n32, cap) ||
+        alloc_array(handle, sl.a_out, s->n32, cap) || alloc_array(handle, sl.b_c, s->n32, cap) ||
+        alloc_array(handle, sl.b_cm, s->h32, 2 * cap) || alloc_array(handle, sl.b_pow, s->h32, 2 * cap) ||
+        alloc_array(handle, sl.b_prod, s->n32 + s->h32, 2 * cap) ||
+        alloc_array(handle, sl.b_sum, s->n32 + s->h32 + 1, cap) || alloc_array(handle, sl.b_out, s->n32, cap))
+      return fail(xmpErrorInvalidCudaMalloc);
+  }
+  if (hipDeviceSynchronize() != hipSuccess) return fail(xmpErrorCuda);
+  s->running = true;
+  s->worker = std::thread(worker_loop, s);
+  *server = s;
+  return xmpErrorSuccess;
+}
+
+xmpError_t XMPAPI xmpServerSubmit(xmpServer_t server, uint32_t kind, uint32_t key, const uint32_t *base,
+                                  const uint32_t *exp, uint64_t tag) {
+  XMP_CHECK_NE(server, nullptr);
+  if (kind > 1 || key >= server->nkeys || !base || (kind == 0 && !exp)) return xmpErrorInvalidParameter;
+  Request r;
+  r.kind = kind;
+  r.key = key;
+  r.tag = tag;
+  r.base.assign(base, base + server->n32);
+  if (kind == 0) r.exp.assign(exp, exp + server->n32);
+  r.arrival = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(server->mu);
+    server->pending.push_back(std::move(r));
+    server->submitted++;
+  }
+  return xmpErrorSuccess;
+}
+
+xmpError_t XMPAPI xmpServerPoll(xmpServer_t server, uint32_t max, uint64_t *tags, uint32_t *results,
+                                double *latency_us, uint32_t *n_out) {
+  XMP_CHECK_NE(server, nullptr);
+  XMP_CHECK_NE(n_out, nullptr);
+  std::lock_guard<std::mutex> lk(server->mu);
+  uint32_t k = 0;
+  while (k < max && !server->done.empty()) {
+    Done &d = server->done.front();
+    if (tags) tags[k] = d.tag;
+    if (results) std::memcpy(results + (size_t)k * server->n32, d.result.data(), sizeof(uint32_t) * server->n32);
+    if (latency_us) latency_us[k] = d.latency_us;
+    server->done.pop_front();
+    k++;
+  }
+  *n_out = k;
+  return server->worker_error;
+}
+
+xmpError_t XMPAPI xmpServerDrain(xmpServer_t server) {
+  XMP_CHECK_NE(server, nullptr);
+  std::unique_lock<std::mutex> lk(server->mu);
+  server->cv.wait(lk, [&] { return server->completed == server->submitted || server->worker_error; });
+  return server->worker_error;
+}
+
+xmpError_t XMPAPI xmpServerDestroy(xmpServer_t server) {
+  XMP_CHECK_NE(server, nullptr);
+  server->running = false;
+  if (server->worker.joinable()) server->worker.join();
+  XMP_SET_DEVICE(server->h);
+  free_server(server);
+  delete server;
+  return xmpErrorSuccess;
+}
+
+xmpError_t XMPAPI xmpServerRunPoisson(xmpServer_t server, double rate_per_s, uint32_t requests, uint64_t seed,
+                                      const uint32_t *kinds, const uint32_t *keys, const uint32_t *bases,
+                                      const uint32_t *exps, uint32_t *results, double *latency_us, double *wall_s) {
+  XMP_CHECK_NE(server, nullptr);
+  if (rate_per_s <= 0 || requests == 0 || !kinds || !keys || !bases || !exps) return xmpErrorInvalidParameter;
+  const uint32_t n32 = server->n32;
+  std::mt19937_64 rng(seed);
+  std::exponential_distribution<double> gap(rate_per_s);
+  auto t0 = Clock::now();
+  double t = 0;
+  for (uint32_t i = 0; i < requests; i++) {
+    t += gap(rng);
+    auto due = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(t));
+    while (Clock::now() < due) {
+    }
+    xmpError_t e = xmpServerSubmit(server, kinds[i], keys[i], bases + (size_t)i * n32, exps + (size_t)i * n32, i);
+    if (e != xmpErrorSuccess) return e;
+  }
+  xmpError_t e = xmpServerDrain(server);
+  auto t1 = Clock::now();
+  if (wall_s) *wall_s = std::chrono::duration<double>(t1 - t0).count();
+  std::vector<uint64_t> tags(requests);
+  std::vector<uint32_t> res((size_t)requests * n32);
+  std::vector<double> lat(requests);
+  uint32_t got = 0;
+  while (got < requests) {
+    uint32_t k = 0;
+    xmpError_t pe = xmpServerPoll(server, requests - got, tags.data() + got, res.data() + (size_t)got * n32,
+                                  lat.data() + got, &k);
+    if (pe != xmpErrorSuccess) return pe;
+    if (k == 0) break;
+    got += k;
+  }
+  for (uint32_t i = 0; i < got; i++) {
+    uint64_t tg = tags[i];
+    if (tg >= requests) continue;
+    if (results) std::memcpy(results + tg * n32, res.data() + (size_t)i * n32, sizeof(uint32_t) * n32);
+    if (latency_us) latency_us[tg] = lat[i];
+  }
+  return e != xmpErrorSuccess ? e : (got == requests ? xmpErrorSuccess : xmpErrorUnspecified);
+}
+
+}  // extern "C"

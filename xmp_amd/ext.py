@@ -102,3 +102,123 @@ def tune(handle, precision, exp_precision, count):
 def tuner_load(json_text):
     from . import check, lib
     check(lib().xmpTunerLoad(json_text.encode()), "xmpTunerLoad")
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("streams", _u32), ("max_batch", _u32), ("hbm_budget", ctypes.c_size_t)]
+
+
+def declare_server(L):
+    E = ctypes.c_int
+    u32p = ctypes.POINTER(_u32)
+    sig = {
+        "xmpServerCreate": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(ServerConfig), _u32, _u32, _vp, _vp, _vp, _vp,
+                            _vp, _vp, _vp],
+        "xmpServerDestroy": [_vp],
+        "xmpServerSubmit": [_vp, _u32, _u32, _vp, _vp, ctypes.c_uint64],
+        "xmpServerPoll": [_vp, _u32, _vp, _vp, _vp, u32p],
+        "xmpServerDrain": [_vp],
+        "xmpServerRunPoisson": [_vp, ctypes.c_double, _u32, ctypes.c_uint64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                ctypes.POINTER(ctypes.c_double)],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = E
+
+
+def make_keys(nkeys, bits, seed):
+    """RSA-style key material with random odd coprime halves p, q (bits/2
+    each, top bit set), n = p*q, random half-size exponents dp, dq and the
+    CRT coefficients cp = q*(q^-1 mod p), cq = p*(p^-1 mod q)."""
+    import math
+    import numpy as np
+    from . import inputs
+    h32, n32 = bits // 64, bits // 32
+    raw = inputs.uniform(seed, 8 * nkeys, h32)
+    keys = {k: np.zeros((nkeys, w), np.uint32) for k, w in
+            (("n", n32), ("p", h32), ("q", h32), ("dp", h32), ("dq", h32), ("cp", n32), ("cq", n32))}
+    to_int = lambda row: int.from_bytes(row.astype("<u4").tobytes(), "little")  # noqa: E731
+    to_row = lambda v, w: np.frombuffer(v.to_bytes(4 * w, "little"), "<u4")  # noqa: E731
+    r = 0
+    for k in range(nkeys):
+        while True:
+            p = to_int(raw[r]) | 1 | (1 << (bits // 2 - 1))
+            q = to_int(raw[r + 1]) | 1 | (1 << (bits // 2 - 1))
+            r += 2
+            if p != q and math.gcd(p, q) == 1:
+                break
+        n = p * q
+        keys["n"][k] = to_row(n, n32)
+        keys["p"][k] = to_row(p, h32)
+        keys["q"][k] = to_row(q, h32)
+        keys["dp"][k] = raw[r]
+        keys["dq"][k] = raw[r + 1]
+        r += 2
+        keys["cp"][k] = to_row(q * pow(q, -1, p), n32)
+        keys["cq"][k] = to_row(p * pow(p, -1, q), n32)
+    return keys
+
+
+class Server:
+    """Continuous-batching powm server over resident keys (xmpServerCreate)."""
+
+    def __init__(self, handle, keys, bits, streams=4, max_batch=1024, hbm_budget=0):
+        import numpy as np
+        from . import check, lib
+        L = lib()
+        declare_server(L)
+        self.handle, self.bits, self.n32 = handle, bits, bits // 32
+        self.keys = {k: np.ascontiguousarray(v, np.uint32) for k, v in keys.items()}
+        cfg = ServerConfig(streams, max_batch, hbm_budget)
+        self.s = _vp()
+        kk = self.keys
+        check(L.xmpServerCreate(handle.h, ctypes.byref(self.s), ctypes.byref(cfg), bits, kk["n"].shape[0],
+                                *[kk[x].ctypes.data for x in ("n", "p", "q", "dp", "dq", "cp", "cq")]),
+              "xmpServerCreate")
+
+    def submit(self, kind, key, base, exp, tag):
+        import numpy as np
+        from . import check, lib
+        b = np.ascontiguousarray(base, np.uint32)
+        e = np.ascontiguousarray(exp, np.uint32)
+        check(lib().xmpServerSubmit(self.s, kind, key, b.ctypes.data, e.ctypes.data, tag), "xmpServerSubmit")
+
+    def drain(self):
+        from . import check, lib
+        check(lib().xmpServerDrain(self.s), "xmpServerDrain")
+
+    def poll(self, max_n=4096):
+        import numpy as np
+        from . import check, lib
+        tags = np.zeros(max_n, np.uint64)
+        res = np.zeros((max_n, self.n32), np.uint32)
+        lat = np.zeros(max_n, np.float64)
+        n = _u32()
+        check(lib().xmpServerPoll(self.s, max_n, tags.ctypes.data, res.ctypes.data, lat.ctypes.data,
+                                  ctypes.byref(n)), "xmpServerPoll")
+        k = n.value
+        return tags[:k], res[:k], lat[:k]
+
+    def run_poisson(self, rate, kinds, keys, bases, exps, seed=1):
+        import numpy as np
+        from . import check, lib
+        n = len(kinds)
+        kinds = np.ascontiguousarray(kinds, np.uint32)
+        keys = np.ascontiguousarray(keys, np.uint32)
+        bases = np.ascontiguousarray(bases, np.uint32)
+        exps = np.ascontiguousarray(exps, np.uint32)
+        res = np.zeros((n, self.n32), np.uint32)
+        lat = np.zeros(n, np.float64)
+        wall = ctypes.c_double()
+        check(lib().xmpServerRunPoisson(self.s, float(rate), n, seed, kinds.ctypes.data, keys.ctypes.data,
+                                        bases.ctypes.data, exps.ctypes.data, res.ctypes.data, lat.ctypes.data,
+                                        ctypes.byref(wall)), "xmpServerRunPoisson")
+        return res, lat, wall.value
+
+    def close(self):
+        from . import lib
+        if self.s:
+            lib().xmpServerDestroy(self.s)
+            self.s = _vp()
+
