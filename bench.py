@@ -149,6 +149,39 @@ def run_serving(h, nkeys, requests, seed):
     return out, (keys, kinds[sample], key_idx[sample], bases[sample], exps[sample], res[sample])
 
 
+def run_only(args):
+    """One kernel, `warmup` + `steps` launches, for counter runs; prints ms per launch."""
+    h = xmp_amd.Handle()
+    stream = hip.stream_create()
+    h.set_stream(stream)
+    lib = xmp_amd.lib()
+    if args.only == "powm":
+        m, b, e, M, B, E = powm_workload(h, 65536, 2048, 2048)
+        out = h.integers(2048, 65536)
+        ctx = Montgomery(h, M, 2048, 65536)
+        fn = lambda: ctx.powm_async(out, B, E, 65536)  # noqa: E731
+        what = "powm_kernel 65,536 x 2048-bit (algorithmic bytes: 4 x 256 B in/out per instance)"
+    elif args.only == "add":
+        n2 = 1 << 20
+        a2, b2 = inputs.ordered_pair(4242, n2, 64)
+        A2, B2 = h.from_array(a2), h.from_array(b2)
+        S2 = h.integers(2048, n2)
+        fn = lambda: lib.xmpIntegersAddAsync(h.h, S2.x, A2.x, B2.x, n2)  # noqa: E731
+        what = "addsub_vec4_kernel 1,048,576 x 2048-bit (algorithmic bytes: 768 B per pair, 805,306,368 B)"
+    else:
+        n2 = 1 << 20
+        src = hip.malloc(n2 * 256, fill=7)   # device-resident compact source (256 MiB)
+        dst = h.integers(2048, n2)
+        fn = lambda: lib.xmpIntegersImportAsync(h.h, dst.x, 64, -1, 4, 0, 0, src, n2)  # noqa: E731
+        what = "import_fast_kernel copy of 268,435,456 B read + 268,435,456 B written"
+    for _ in range(args.warmup):
+        fn()
+    hip.stream_sync(stream)
+    ms = time_async(fn, stream, max(1, args.steps))
+    print(json.dumps({"only": args.only, "what": what, "ms_per_launch": round(ms, 4)}), flush=True)
+    h.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,11 +189,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["powm", "add", "copy"],
+                    help="run just one kernel (for counter profiling): the 2048-bit powm kernel with resident "
+                         "constants, the 2048-bit add, or a plain 256 MiB import copy for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the instances are independent, so N GPUs would each run this "
               "same per-GPU workload on their own share; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.only:
+        run_only(args)
+        return
 
     N, BITS = 65536, 2048
     h = xmp_amd.Handle()

@@ -239,7 +239,7 @@ xmpError_t XMPAPI xmpTunerRun(xmpHandle_t handle, uint32_t precision, uint32_t e
       args.start = 0;
       args.count = count;
       float best_ms = 1e30f;
-      for (int rep = 0; rep < 3; rep++) {
+      for (int rep = 0; rep < 2; rep++) {
         (void)hipEventRecord(ev0, handle->stream);
         (void)launch_powm_variant(vi, args, handle->stream);
         (void)hipEventRecord(ev1, handle->stream);
@@ -256,6 +256,8 @@ xmpError_t XMPAPI xmpTunerRun(xmpHandle_t handle, uint32_t precision, uint32_t e
                     "{\"precision\": %u, \"exp_bits\": %u, \"count\": %u, \"lanes\": %d, \"digits\": %d, "
                     "\"radix\": %d, \"window\": %u, \"ms\": %.4f, \"powm_per_s\": %.1f}",
                     precision, exp_precision, count, v.lanes, v.digits, v.radix, w, best_ms, rate);
+      if (!out.empty()) out += ",\n";
+      out += buf;
       if (rate > best_rate) {
         best_rate = rate;
         best_entry = buf;
@@ -266,7 +268,6 @@ xmpError_t XMPAPI xmpTunerRun(xmpHandle_t handle, uint32_t precision, uint32_t e
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
   cleanup();
-  if (!best_entry.empty()) out = best_entry;
   if (out.size() + 1 > json_bytes) return xmpErrorInvalidParameter;
   std::memcpy(json, out.c_str(), out.size() + 1);
   if (measured) *measured = nmeasured;

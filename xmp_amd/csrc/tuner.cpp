@@ -1,6 +1,8 @@
 // tuner.cpp -- runtime choice of powm layout: measured table, else model.
 #include "tuner.hpp"
 
+#include <dlfcn.h>
+
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -54,12 +56,23 @@ void parse_table(const std::string &text, std::vector<TunedEntry> *dst) {
   }
 }
 
+// the measured table shipped next to the library (xmp_amd/tune_gfx950.json),
+// overridden by $XMP_TUNE_FILE
+std::string default_table_path() {
+  Dl_info info;
+  if (dladdr((void *)&default_table_path, &info) == 0 || !info.dli_fname) return std::string();
+  std::string so(info.dli_fname);
+  size_t slash = so.rfind('/');
+  return (slash == std::string::npos ? std::string(".") : so.substr(0, slash)) + "/tune_gfx950.json";
+}
+
 void ensure_loaded() {
   if (g_loaded) return;
   g_loaded = true;
-  const char *path = std::getenv("XMP_TUNE_FILE");
-  if (!path) return;
-  FILE *f = std::fopen(path, "rb");
+  const char *env = std::getenv("XMP_TUNE_FILE");
+  std::string path = env ? std::string(env) : default_table_path();
+  if (path.empty()) return;
+  FILE *f = std::fopen(path.c_str(), "rb");
   if (!f) return;
   std::string text;
   char buf[4096];
@@ -124,17 +137,20 @@ xmpError_t choose_powm(xmpHandle_t h, uint32_t precision, uint32_t exp_bits, uin
     if (alg == xmpAlgorithmDistributedMP) return v.lanes > 1;
     return true;
   };
-  // measured entries for this precision: nearest count (log scale), same exponent size preferred
+  // measured entries for this precision: the batch size nearest the request
+  // (log scale, same exponent size preferred), then the fastest allowed layout
+  // measured at that size
   const TunedEntry *best = nullptr;
-  double best_d = 1e30;
+  double best_d = 1e30, best_rate = -1;
   for (const TunedEntry &e : g_table) {
     if (e.precision != precision) continue;
     int vi = powm_find_variant(e.lanes, e.digits, e.radix);
     if (vi < 0 || !allowed(powm_variant(vi))) continue;
     double d = std::fabs(std::log((double)e.count + 1) - std::log((double)count + 1)) +
                (e.exp_bits == exp_bits ? 0.0 : 10.0);
-    if (d < best_d) {
+    if (d < best_d - 1e-9 || (std::fabs(d - best_d) <= 1e-9 && e.rate > best_rate)) {
       best_d = d;
+      best_rate = e.rate;
       best = &e;
     }
   }

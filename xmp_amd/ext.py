@@ -93,9 +93,9 @@ class Batch:
 def tune(handle, precision, exp_precision, count):
     """Run the on-device tuner; returns the best entry as JSON text."""
     from . import check, lib
-    buf = ctypes.create_string_buffer(4096)
+    buf = ctypes.create_string_buffer(1 << 18)
     n = _u32()
-    check(lib().xmpTunerRun(handle.h, precision, exp_precision, count, buf, 4096, ctypes.byref(n)), "xmpTunerRun")
+    check(lib().xmpTunerRun(handle.h, precision, exp_precision, count, buf, len(buf), ctypes.byref(n)), "xmpTunerRun")
     return buf.value.decode(), n.value
 
 

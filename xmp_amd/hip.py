@@ -32,6 +32,9 @@ def hip():
         H.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), _vp, _vp]
         H.hipDeviceSynchronize.argtypes = []
         H.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        H.hipMalloc.argtypes = [ctypes.POINTER(_vp), ctypes.c_size_t]
+        H.hipFree.argtypes = [_vp]
+        H.hipMemset.argtypes = [_vp, ctypes.c_int, ctypes.c_size_t]
     return _hip
 
 
@@ -77,3 +80,15 @@ class Event:
         ms = ctypes.c_float()
         _ok(hip().hipEventElapsedTime(ctypes.byref(ms), self.e, later.e), "hipEventElapsedTime")
         return ms.value
+
+
+def malloc(nbytes, fill=None):
+    p = _vp()
+    _ok(hip().hipMalloc(ctypes.byref(p), nbytes), "hipMalloc")
+    if fill is not None:
+        _ok(hip().hipMemset(p, fill, nbytes), "hipMemset")
+    return p.value
+
+
+def free(ptr):
+    _ok(hip().hipFree(_vp(ptr)), "hipFree")
