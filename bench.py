@@ -31,6 +31,10 @@ HBM_PEAK_GBS = 8000.0
 # at 2.4 GHz = the published 78.6 TFLOP/s FP64 vector rate of the same datapath;
 # a multiply-add counts as 2 integer operations
 VALU_PEAK_TOPS = 78.6
+# HBM bytes per launch of powm_kernel<2,37,28> on 65,536 x 2048-bit instances:
+# rocprofv3 FETCH_SIZE x 2 (read factor calibrated on a copy kernel of known
+# size) + WRITE_SIZE, from two --pmc runs of `bench.py --only powm`
+POWM_TRAFFIC_BYTES = 10545780576
 
 
 def canonical_powm_macs(bits, exp_bits):
@@ -155,6 +159,11 @@ def run_only(args):
     stream = hip.stream_create()
     h.set_stream(stream)
     lib = xmp_amd.lib()
+    if args.only == "mixed":
+        res, _ = run_mixed(h, stream, 262144, 777, reps=max(1, args.steps))
+        print(json.dumps(res), flush=True)
+        h.close()
+        return
     if args.only == "powm":
         m, b, e, M, B, E = powm_workload(h, 65536, 2048, 2048)
         out = h.integers(2048, 65536)
@@ -189,7 +198,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["powm", "add", "copy"],
+    ap.add_argument("--only", choices=["powm", "add", "copy", "mixed"],
                     help="run just one kernel (for counter profiling): the 2048-bit powm kernel with resident "
                          "constants, the 2048-bit add, or a plain 256 MiB import copy for counter calibration")
     args = ap.parse_args()
@@ -241,6 +250,11 @@ def main():
                 "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
                 "kernel": "powm_kernel<%d,%d,%d> window %d" % (lanes, digits, radix, wbits),
                 "kernel_ms": round(kern_ms, 4)}
+    # the issue rate a pure v_mad_u64_u32 stream reaches at 2 and 8 waves per SIMD
+    from xmp_amd.ext import probe_mad_rate
+    roofline["measured_mad_per_clk_per_cu"] = {"2_waves": round(probe_mad_rate(h, 2), 2),
+                                               "8_waves": round(probe_mad_rate(h, 8), 2), "full_rate": 64}
+    roofline["traffic"] = POWM_TRAFFIC_BYTES if (lanes, digits, radix) == (2, 37, 28) else None
     log("powm kernel %.3f ms -> %.2f TOP/s (%.1f%% of %.1f)" % (kern_ms, achieved, 100 * achieved / VALU_PEAK_TOPS,
                                                             VALU_PEAK_TOPS))
     ctx.close()

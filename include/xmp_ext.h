@@ -107,6 +107,13 @@ xmpError_t XMPAPI xmpTunerRun(xmpHandle_t handle, uint32_t precision, uint32_t e
 /* Load a tuning table (JSON text as written by xmpTunerRun or the tune tool). */
 xmpError_t XMPAPI xmpTunerLoad(const char *json);
 
+/* ---------------- diagnostics ---------------- */
+/* Issue-rate probe: independent v_mad_u64_u32 streams on every CU with
+ * `waves_per_simd` waves per SIMD (1..8); returns multiply-adds per clock per
+ * CU (64 = one per lane per clock) and the probe's kernel time. */
+xmpError_t XMPAPI xmpProbeMadRate(xmpHandle_t handle, uint32_t waves_per_simd, double *mads_per_clk_per_cu,
+                                  double *seconds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,7 +29,7 @@ SOURCES = [
     "k_arith.hip",
     "k_div.hip",
     "k_powm.hip",
-    "k_batch.hip",
+    "k_probe.hip",
 ]
 
 COMMON = [

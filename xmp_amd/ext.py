@@ -20,6 +20,7 @@ def declare(L):
         "xmpBatchLaunchPlainAsync": [_vp, _vp],
         "xmpTunerRun": [_vp, _u32, _u32, _u32, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(_u32)],
         "xmpTunerLoad": [ctypes.c_char_p],
+        "xmpProbeMadRate": [_vp, _u32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     }
     for name, args in sig.items():
         f = getattr(L, name, None)
@@ -97,6 +98,14 @@ def tune(handle, precision, exp_precision, count):
     n = _u32()
     check(lib().xmpTunerRun(handle.h, precision, exp_precision, count, buf, len(buf), ctypes.byref(n)), "xmpTunerRun")
     return buf.value.decode(), n.value
+
+
+def probe_mad_rate(handle, waves_per_simd):
+    """Measured v_mad_u64_u32 multiply-adds per clock per CU (xmpProbeMadRate)."""
+    from . import check, lib
+    r, t = ctypes.c_double(), ctypes.c_double()
+    check(lib().xmpProbeMadRate(handle.h, waves_per_simd, ctypes.byref(r), ctypes.byref(t)), "xmpProbeMadRate")
+    return r.value
 
 
 def tuner_load(json_text):
