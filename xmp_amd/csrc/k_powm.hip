@@ -22,7 +22,8 @@
 //  * an instance spans L lanes of the wave (L = 1, 2, 4, 8), each holding K
 //    digits of the multiplicand, the modulus and the accumulator in
 //    registers.  Per CIOS step the quotient digit is broadcast from the
-//    group's first lane and one 64-bit accumulator moves one lane down;
+//    group's first lane and one 28-bit digit moves one lane down, both with
+//    DPP moves;
 //  * the multiplier (the current value when squaring, a table entry when
 //    multiplying) is staged in LDS and read four digits at a time with an
 //    LDS broadcast;
@@ -38,27 +39,54 @@ namespace xmp {
 
 namespace {
 
+// Cross-lane moves inside an instance's group of L lanes (groups are aligned
+// within 16-lane DPP rows), done with DPP moves on the VALU instead of LDS
+// permutes:
+//   bcast0    -- the group's first lane's value to every lane of the group
+//   from_next -- lane l receives lane l+1's value, the group's last lane 0
+//   from_prev -- lane l receives lane l-1's value, the group's first lane 0
+// DPP controls: quad_perm [0,0,2,2] = 0xA0, [0,0,0,0] = 0x00, row_shl:1 =
+// 0x101, row_shr:1 = 0x111, row_shr:4 = 0x114, row_newbcast:0 = 0x150 (lane 0
+// of each row to the whole row).
 template <int L>
 struct Group {
   int l;   // lane within the instance group
   __device__ __forceinline__ explicit Group(int lane) : l(lane & (L - 1)) {}
   __device__ __forceinline__ uint32_t bcast0(uint32_t x) const {
-    if (L == 1) return x;
-    return (uint32_t)__shfl((int)x, 0, L);
+    if constexpr (L == 1) {
+      return x;
+    } else if constexpr (L == 2) {
+      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xA0, 0xF, 0xF, false);
+    } else if constexpr (L == 4) {
+      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x00, 0xF, 0xF, false);
+    } else if constexpr (L == 8) {
+      uint32_t y = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x00, 0xF, 0xF, false);
+      uint32_t z = (uint32_t)__builtin_amdgcn_mov_dpp((int)y, 0x114, 0xF, 0xF, true);
+      return (l & 4) ? z : y;
+    } else {
+      static_assert(L == 16, "groups are at most one DPP row");
+      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x150, 0xF, 0xF, false);
+    }
   }
-  __device__ __forceinline__ uint64_t from_next(uint64_t x) const {
-    if (L == 1) return 0;
-    uint32_t lo = (uint32_t)__shfl_down((int)(uint32_t)x, 1, L);
-    uint32_t hi = (uint32_t)__shfl_down((int)(uint32_t)(x >> 32), 1, L);
-    uint64_t v = ((uint64_t)hi << 32) | lo;
-    return l == L - 1 ? 0ull : v;
+  __device__ __forceinline__ uint32_t from_next(uint32_t x) const {
+    if constexpr (L == 1) {
+      return 0;
+    } else {
+      uint32_t v = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x101, 0xF, 0xF, true);
+      if constexpr (L == 16) return v;   // the row's last lane already reads 0
+      return l == L - 1 ? 0u : v;
+    }
   }
   __device__ __forceinline__ uint64_t from_prev(uint64_t x) const {
-    if (L == 1) return 0;
-    uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)x, 1, L);
-    uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(x >> 32), 1, L);
-    uint64_t v = ((uint64_t)hi << 32) | lo;
-    return l == 0 ? 0ull : v;
+    if constexpr (L == 1) {
+      return 0;
+    } else {
+      uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)x, 0x111, 0xF, 0xF, true);
+      uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(x >> 32), 0x111, 0xF, 0xF, true);
+      uint64_t v = ((uint64_t)hi << 32) | lo;
+      if constexpr (L == 16) return v;   // the row's first lane already reads 0
+      return l == 0 ? 0ull : v;
+    }
   }
   __device__ __forceinline__ bool any(bool p) const {
     if (L == 1) return p;
@@ -79,16 +107,22 @@ struct Mont {
 
   __device__ __forceinline__ explicit Mont(int lane) : g(lane) {}
 
+  // One CIOS step: t = (t + a*b_i + q*m) / 2^D with q making the lowest digit
+  // vanish.  Lane l's first accumulator f (position l*K) is split as
+  // f = f_lo + 2^D f_hi: f_lo moves to the previous lane's top position and
+  // f_hi stays at this lane's new position 0.  For the group's first lane
+  // f_lo = 0, so the code is the same on every lane and only a 28-bit value
+  // crosses lanes.
   __device__ __forceinline__ void step(uint64_t (&t)[K], const uint32_t (&a)[K], uint32_t bi) const {
     uint64_t f = t[0] + (uint64_t)a[0] * bi;
     uint32_t q = ((uint32_t)f * np0) & MASK;
     q = g.bcast0(q);
     f += (uint64_t)q * m[0];
-    uint64_t up = g.from_next(f);
+    const uint32_t up = g.from_next((uint32_t)f & MASK);
 #pragma unroll
     for (int j = 1; j < K; j++) t[j - 1] = t[j] + (uint64_t)a[j] * bi + (uint64_t)q * m[j];
     t[K - 1] = up;
-    t[0] += (g.l == 0) ? (f >> D) : 0ull;
+    t[0] += f >> D;
   }
 
   // r = a * b / R (mod m), r < 2m, normalised digits.  b: group's ND digits in LDS.
@@ -308,7 +342,7 @@ const VariantEntry kVariants[] = {
     XMP_VARIANT(8, 19, 27),  // 4104 : 4096-bit on eight lanes
     XMP_VARIANT(8, 29, 27),  // 6264 : 6144-bit
     XMP_VARIANT(8, 38, 27),  // 8208 : 8192-bit
-    XMP_VARIANT(16, 19, 27), // 8208 : 8192-bit on sixteen lanes
+    XMP_VARIANT(16, 19, 27), // 8208 : 8192-bit on sixteen lanes (one DPP row)
 };
 #undef XMP_VARIANT
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
