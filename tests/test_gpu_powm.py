@@ -28,12 +28,38 @@ def _powm_case(handle, seed, n, bits, exp_bits=None, reduce_base=True, count_mod
     return Out.get(), b, e, m
 
 
-@pytest.mark.parametrize("bits", [64, 128, 256, 320, 512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192])
+@pytest.mark.parametrize("bits", [64, 96, 128, 160, 256, 288, 320, 384, 512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192])
 def test_powm_sizes(handle, bits):
     n = 128 if bits <= 2048 else 64
     got, b, e, m = _powm_case(handle, 100 + bits, n, bits)
     want = gmp(O.GO_POWM, n, bits // 32, b, e, m)
     assert_rows_equal(got, want, "powm %d" % bits)
+
+
+@pytest.mark.parametrize("bits,exp_bits,n", [(12288, 128, 6), (16384, 256, 4), (16384, 16384, 2), (32768, 64, 3)])
+def test_powm_above_8192_bits(handle, bits, exp_bits, n):
+    """Moduli above 8192 bits (the reference's digit-serial fallback range,
+    powm_implementations.h:154-286): half-wave and whole-wave layouts."""
+    got, b, e, m = _powm_case(handle, 900 + bits + exp_bits, n, bits, exp_bits=exp_bits)
+    assert_rows_equal(got, gmp(O.GO_POWM, n, bits // 32, b, e, m), "powm %d-bit, %d-bit exponent" % (bits, exp_bits))
+
+
+def test_powm_above_8192_bits_special_values(handle):
+    """e = 0, b = 0, b = m - 1 and a small modulus in a 16384-bit container."""
+    limbs, n = 512, 4
+    m = inputs.odd_moduli(951, n, limbs)
+    b = inputs.uniform(952, n, limbs)
+    e = inputs.uniform(953, n, 4)
+    e[0] = 0
+    b[1] = 0
+    b[2] = m[2]
+    b[2, 0] -= 1
+    m[3] = 0
+    m[3, 0] = 0xFFFFFFFB   # 2^32 - 5, prime
+    M, B, E = handle.from_array(m), handle.from_array(b), handle.from_array(e)
+    Out = handle.integers(limbs * 32, n)
+    handle.powm(Out, B, E, M)
+    assert_rows_equal(Out.get(), gmp(O.GO_POWM, n, limbs, b, e, m), "special values above 8192 bits")
 
 
 @pytest.mark.parametrize("bits", [1024, 2048])

@@ -29,45 +29,60 @@ __device__ __forceinline__ void addsub_limb(uint32_t &r, uint32_t a, uint32_t b,
 }
 
 // Fast path: no index arrays and both inputs hold at least `count` instances.
+// Operands are streamed once, so loads and stores are non-temporal (they do
+// not displace cached data); when both inputs are at least as long as the
+// output (FULL) the limb loop carries no zero-extension tests.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <bool NT>
+__device__ __forceinline__ u32x4 ld4(const uint32_t *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+  else return *reinterpret_cast<const u32x4 *>(p);
+}
+
+__device__ __forceinline__ void st4(uint32_t *p, u32x4 v) { __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(p)); }
+
 template <bool SUB>
-__global__ void __launch_bounds__(256) addsub_vec4_kernel(uint32_t *__restrict__ s, uint32_t s_len, uint32_t s_stride,
-                                                          const uint32_t *__restrict__ a, uint32_t a_len,
-                                                          uint32_t a_stride, const uint32_t *__restrict__ b,
-                                                          uint32_t b_len, uint32_t b_stride, uint32_t count) {
+__device__ __forceinline__ u32x4 addsub4(u32x4 a, u32x4 b, uint32_t (&c)[4]) {
+  uint32_t r0, r1, r2, r3;
+  addsub_limb<SUB>(r0, a.x, b.x, c[0]);
+  addsub_limb<SUB>(r1, a.y, b.y, c[1]);
+  addsub_limb<SUB>(r2, a.z, b.z, c[2]);
+  addsub_limb<SUB>(r3, a.w, b.w, c[3]);
+  return u32x4{r0, r1, r2, r3};
+}
+
+template <bool SUB, bool FULL>
+__global__ void __launch_bounds__(64) addsub_vec4_kernel(uint32_t *__restrict__ s, uint32_t s_len, uint32_t s_stride,
+                                                         const uint32_t *__restrict__ a, uint32_t a_len,
+                                                         uint32_t a_stride, const uint32_t *__restrict__ b,
+                                                         uint32_t b_len, uint32_t b_stride, uint32_t count) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t nvec = count / 4;
   if (t < nvec) {
     uint32_t i = 4 * t;
-    uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+    uint32_t c[4] = {0, 0, 0, 0};
     uint32_t j = 0;
+    const u32x4 zero = {0, 0, 0, 0};
     for (; j + 8 <= s_len; j += 8) {
-      uint4 av[8], bv[8];
+      u32x4 av[8], bv[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) {
-        av[k] = (j + k < a_len) ? *reinterpret_cast<const uint4 *>(a + (size_t)(j + k) * a_stride + i)
-                                : make_uint4(0, 0, 0, 0);
-        bv[k] = (j + k < b_len) ? *reinterpret_cast<const uint4 *>(b + (size_t)(j + k) * b_stride + i)
-                                : make_uint4(0, 0, 0, 0);
+        if (FULL) {
+          av[k] = ld4<true>(a + (size_t)(j + k) * a_stride + i);
+          bv[k] = ld4<true>(b + (size_t)(j + k) * b_stride + i);
+        } else {
+          av[k] = (j + k < a_len) ? ld4<true>(a + (size_t)(j + k) * a_stride + i) : zero;
+          bv[k] = (j + k < b_len) ? ld4<true>(b + (size_t)(j + k) * b_stride + i) : zero;
+        }
       }
 #pragma unroll
-      for (int k = 0; k < 8; k++) {
-        uint4 r;
-        addsub_limb<SUB>(r.x, av[k].x, bv[k].x, c0);
-        addsub_limb<SUB>(r.y, av[k].y, bv[k].y, c1);
-        addsub_limb<SUB>(r.z, av[k].z, bv[k].z, c2);
-        addsub_limb<SUB>(r.w, av[k].w, bv[k].w, c3);
-        *reinterpret_cast<uint4 *>(s + (size_t)(j + k) * s_stride + i) = r;
-      }
+      for (int k = 0; k < 8; k++) st4(s + (size_t)(j + k) * s_stride + i, addsub4<SUB>(av[k], bv[k], c));
     }
     for (; j < s_len; j++) {
-      uint4 av = (j < a_len) ? *reinterpret_cast<const uint4 *>(a + (size_t)j * a_stride + i) : make_uint4(0, 0, 0, 0);
-      uint4 bv = (j < b_len) ? *reinterpret_cast<const uint4 *>(b + (size_t)j * b_stride + i) : make_uint4(0, 0, 0, 0);
-      uint4 r;
-      addsub_limb<SUB>(r.x, av.x, bv.x, c0);
-      addsub_limb<SUB>(r.y, av.y, bv.y, c1);
-      addsub_limb<SUB>(r.z, av.z, bv.z, c2);
-      addsub_limb<SUB>(r.w, av.w, bv.w, c3);
-      *reinterpret_cast<uint4 *>(s + (size_t)j * s_stride + i) = r;
+      u32x4 av = (FULL || j < a_len) ? ld4<true>(a + (size_t)j * a_stride + i) : zero;
+      u32x4 bv = (FULL || j < b_len) ? ld4<true>(b + (size_t)j * b_stride + i) : zero;
+      st4(s + (size_t)j * s_stride + i, addsub4<SUB>(av, bv, c));
     }
   } else if (t < nvec + (count & 3)) {
     uint32_t i = 4 * nvec + (t - nvec);
@@ -80,6 +95,18 @@ __global__ void __launch_bounds__(256) addsub_vec4_kernel(uint32_t *__restrict__
       s[(size_t)j * s_stride + i] = r;
     }
   }
+}
+
+template <bool SUB>
+void launch_addsub_vec4(const OutOperand &s, const Operand &a, const Operand &b, uint32_t count, hipStream_t stream) {
+  const uint32_t threads = count / 4 + (count & 3);
+  const dim3 grid(div_round_up(threads, 64)), block(64);
+  if (a.len >= s.len && b.len >= s.len)
+    hipLaunchKernelGGL((addsub_vec4_kernel<SUB, true>), grid, block, 0, stream, s.data, s.len, s.stride, a.data,
+                       a.len, a.stride, b.data, b.len, b.stride, count);
+  else
+    hipLaunchKernelGGL((addsub_vec4_kernel<SUB, false>), grid, block, 0, stream, s.data, s.len, s.stride, a.data,
+                       a.len, a.stride, b.data, b.len, b.stride, count);
 }
 
 // General path: one instance per lane with index arrays and broadcast.
@@ -218,9 +245,7 @@ hipError_t launch_mul_rows(OutOperand p, Operand a, Operand b, uint32_t count, h
 
 hipError_t launch_add(OutOperand s, Operand a, Operand b, uint32_t count, hipStream_t stream) {
   if (!s.idx && !a.idx && !b.idx && a.count >= count && b.count >= count) {
-    uint32_t threads = count / 4 + (count & 3);
-    hipLaunchKernelGGL(addsub_vec4_kernel<false>, dim3(div_round_up(threads, 256)), dim3(256), 0, stream, s.data,
-                       s.len, s.stride, a.data, a.len, a.stride, b.data, b.len, b.stride, count);
+    launch_addsub_vec4<false>(s, a, b, count, stream);
   } else {
     hipLaunchKernelGGL(addsub_general_kernel<false>, dim3(div_round_up(count, 256)), dim3(256), 0, stream, s, a, b,
                        count);
@@ -230,9 +255,7 @@ hipError_t launch_add(OutOperand s, Operand a, Operand b, uint32_t count, hipStr
 
 hipError_t launch_sub(OutOperand d, Operand a, Operand b, uint32_t count, hipStream_t stream) {
   if (!d.idx && !a.idx && !b.idx && a.count >= count && b.count >= count) {
-    uint32_t threads = count / 4 + (count & 3);
-    hipLaunchKernelGGL(addsub_vec4_kernel<true>, dim3(div_round_up(threads, 256)), dim3(256), 0, stream, d.data,
-                       d.len, d.stride, a.data, a.len, a.stride, b.data, b.len, b.stride, count);
+    launch_addsub_vec4<true>(d, a, b, count, stream);
   } else {
     hipLaunchKernelGGL(addsub_general_kernel<true>, dim3(div_round_up(count, 256)), dim3(256), 0, stream, d, a, b,
                        count);

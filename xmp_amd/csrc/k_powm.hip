@@ -11,7 +11,8 @@
 // (nearly) full rate but every carry instruction costs as much as the
 // multiply:
 //
-//  * operands are held as radix-2^D digits (D = 28, or 27 above 127 digits)
+//  * operands are held as radix-2^D digits (D = 28; 27 above 127 digits; 26
+//    above 511)
 //    and the CIOS accumulator as 64-bit "lazy" sums: each partial product is a
 //    single v_mad_u64_u32 into its 64-bit accumulator and no carry is
 //    propagated inside the multiplication (2*ND products of < 2^(2D) each fit
@@ -19,7 +20,7 @@
 //  * R = 2^(D*ND) >= 4m, so every product of inputs < 2m is < 2m again
 //    (Walter's bound) and no conditional subtraction is needed until the
 //    final conversion;
-//  * an instance spans L lanes of the wave (L = 1, 2, 4, 8), each holding K
+//  * an instance spans L lanes of the wave (L = 1 ... 64), each holding K
 //    digits of the multiplicand, the modulus and the accumulator in
 //    registers.  Per CIOS step the quotient digit is broadcast from the
 //    group's first lane and one 28-bit digit moves one lane down, both with
@@ -32,6 +33,8 @@
 // The exponent is scanned over all of its precision's bits with a fixed
 // window (the reference's constant-time schedule, powm.h:94-98), and the
 // result is reduced to the canonical value < m (GMP semantics).
+#include <utility>
+
 #include "devmp.hpp"
 #include "powm.hpp"
 
@@ -39,17 +42,23 @@ namespace xmp {
 
 namespace {
 
-// Cross-lane moves inside an instance's group of L lanes (groups are aligned
-// within 16-lane DPP rows), done with DPP moves on the VALU instead of LDS
-// permutes:
+// Cross-lane moves inside an instance's group of L lanes (groups are aligned:
+// L <= 16 stays inside a 16-lane DPP row), done with DPP moves on the VALU
+// instead of LDS permutes:
 //   bcast0    -- the group's first lane's value to every lane of the group
 //   from_next -- lane l receives lane l+1's value, the group's last lane 0
 //   from_prev -- lane l receives lane l-1's value, the group's first lane 0
 // DPP controls: quad_perm [0,0,2,2] = 0xA0, [0,0,0,0] = 0x00, row_shl:1 =
 // 0x101, row_shr:1 = 0x111, row_shr:4 = 0x114, row_newbcast:0 = 0x150 (lane 0
-// of each row to the whole row).
+// of each row to the whole row), row_bcast:15 = 0x142 (lane 15 of each row to
+// the next row), wave_shl:1 = 0x130 and wave_shr:1 = 0x138 (whole-wave shifts,
+// for groups that span rows).  A 64-lane group reads its first lane with
+// readfirstlane, which leaves the quotient digit in a scalar register.
 template <int L>
 struct Group {
+  static_assert(L == 1 || L == 2 || L == 4 || L == 8 || L == 16 || L == 32 || L == 64, "group size");
+  static constexpr int SHL1 = L <= 16 ? 0x101 : 0x130;
+  static constexpr int SHR1 = L <= 16 ? 0x111 : 0x138;
   int l;   // lane within the instance group
   __device__ __forceinline__ explicit Group(int lane) : l(lane & (L - 1)) {}
   __device__ __forceinline__ uint32_t bcast0(uint32_t x) const {
@@ -63,17 +72,22 @@ struct Group {
       uint32_t y = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x00, 0xF, 0xF, false);
       uint32_t z = (uint32_t)__builtin_amdgcn_mov_dpp((int)y, 0x114, 0xF, 0xF, true);
       return (l & 4) ? z : y;
-    } else {
-      static_assert(L == 16, "groups are at most one DPP row");
+    } else if constexpr (L == 16) {
       return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x150, 0xF, 0xF, false);
+    } else if constexpr (L == 32) {
+      // every row takes its lane 0; rows 1 and 3 then take lane 15 of rows 0 and 2
+      uint32_t y = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x150, 0xF, 0xF, false);
+      return (uint32_t)__builtin_amdgcn_update_dpp((int)y, (int)y, 0x142, 0xA, 0xF, false);
+    } else {
+      return (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
     }
   }
   __device__ __forceinline__ uint32_t from_next(uint32_t x) const {
     if constexpr (L == 1) {
       return 0;
     } else {
-      uint32_t v = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x101, 0xF, 0xF, true);
-      if constexpr (L == 16) return v;   // the row's last lane already reads 0
+      uint32_t v = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, SHL1, 0xF, 0xF, true);
+      if constexpr (L == 16 || L == 64) return v;   // the row's / wave's last lane already reads 0
       return l == L - 1 ? 0u : v;
     }
   }
@@ -81,10 +95,10 @@ struct Group {
     if constexpr (L == 1) {
       return 0;
     } else {
-      uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)x, 0x111, 0xF, 0xF, true);
-      uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(x >> 32), 0x111, 0xF, 0xF, true);
+      uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)x, SHR1, 0xF, 0xF, true);
+      uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(x >> 32), SHR1, 0xF, 0xF, true);
       uint64_t v = ((uint64_t)hi << 32) | lo;
-      if constexpr (L == 16) return v;   // the row's first lane already reads 0
+      if constexpr (L == 16 || L == 64) return v;   // the row's / wave's first lane already reads 0
       return l == 0 ? 0ull : v;
     }
   }
@@ -143,6 +157,11 @@ struct Mont {
     normalize(r, t);
   }
 
+  // Resolve the lazy sums into D-bit digits.  Each lane first carries through
+  // its own K digits; the lane carries (< 2^(64-D)) then ripple up the group
+  // one lane per round.  After the first round a lane passes a carry on only
+  // if its digits were all ones, so the rounds stop as soon as no lane of the
+  // wave holds a carry (almost always after one round).
   __device__ __forceinline__ void normalize(uint32_t (&r)[K], const uint64_t (&t)[K]) const {
     uint64_t c = 0;
 #pragma unroll
@@ -151,16 +170,65 @@ struct Mont {
       r[k] = (uint32_t)x & MASK;
       c = x >> D;
     }
+    if constexpr (L > 1) {
+      for (int round = 1; round < L; round++) {
+        c = g.from_prev(c);
+        if (round > 1 && __builtin_amdgcn_ballot_w64(c != 0) == 0) break;
+        // c < 2^(64-D): past the first digit the carry fits in 32 bits
+        uint64_t x = (uint64_t)r[0] + c;
+        r[0] = (uint32_t)x & MASK;
+        uint32_t c32 = (uint32_t)(x >> D);
 #pragma unroll
-    for (int round = 1; round < L; round++) {
-      c = g.from_prev(c);
-#pragma unroll
-      for (int k = 0; k < K; k++) {
-        uint64_t x = (uint64_t)r[k] + c;
-        r[k] = (uint32_t)x & MASK;
-        c = x >> D;
+        for (int k = 1; k < K; k++) {
+          uint32_t y = r[k] + c32;
+          r[k] = y & MASK;
+          c32 = y >> D;
+        }
+        c = c32;
       }
     }
+  }
+
+  // r = a^2 / R (mod m) for one-lane instances: the CIOS steps with the
+  // multiplier digits taken from the lane's own registers and the symmetric
+  // partial products folded, so step i adds a_i^2 at position i and 2 a_i a_j
+  // for j > i.  A position's products all arrive no later than the step that
+  // retires it (pair (i, j) lands at i + j >= i), so the quotient digits are
+  // those of the plain product.  ND(ND+1)/2 + ND^2 multiply-adds instead of
+  // 2 ND^2.  Bound per accumulator: (ND/2+1) doubled products < 2^(2D+1) plus
+  // ND products < 2^(2D), the same 2^(2D)(2ND+2) as the plain product.
+  __device__ __forceinline__ void sqr(uint32_t (&r)[K], const uint32_t (&a)[K]) const {
+    static_assert(L == 1, "symmetric squaring needs the whole instance in one lane");
+    uint64_t t[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) t[k] = 0;
+    // one instantiation per step, so every register index is a constant
+    sqr_steps(t, a, std::make_integer_sequence<int, K>());
+    normalize(r, t);
+  }
+
+  template <int... I>
+  __device__ __forceinline__ void sqr_steps(uint64_t (&t)[K], const uint32_t (&a)[K],
+                                            std::integer_sequence<int, I...>) const {
+    (sqr_step<I>(t, a), ...);
+  }
+
+  template <int I>
+  __device__ __forceinline__ void sqr_step(uint64_t (&t)[K], const uint32_t (&a)[K]) const {
+    const uint32_t ai = a[I], ai2 = a[I] << 1;
+    uint64_t f = t[0];
+    if constexpr (I == 0) f += (uint64_t)ai * ai;
+    const uint32_t q = ((uint32_t)f * np0) & MASK;
+    f += (uint64_t)q * m[0];
+#pragma unroll
+    for (int j = 1; j < K; j++) {
+      uint64_t x = t[j] + (uint64_t)q * m[j];
+      if (j == I) x += (uint64_t)ai * a[j];
+      else if (j > I) x += (uint64_t)ai2 * a[j];
+      t[j - 1] = x;
+    }
+    t[K - 1] = 0;
+    t[0] += f >> D;
   }
 };
 
@@ -262,11 +330,15 @@ __device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, u
     pos -= wbits;
     idx = exp_bits(args.exp, ei, pos, wbits);
     for (uint32_t s = 0; s < wbits; s++) {
+      if constexpr (L == 1) {
+        mont.sqr(a, a);
+      } else {
 #pragma unroll
-      for (int k = 0; k < K; k++) b[l * K + k] = a[k];
-      __syncthreads();
-      mont.mul(a, a, b);
-      __syncthreads();
+        for (int k = 0; k < K; k++) b[l * K + k] = a[k];
+        __syncthreads();
+        mont.mul(a, a, b);
+        __syncthreads();
+      }
     }
 #pragma unroll
     for (int k = 0; k < K; k++) b[l * K + k] = *tab_at(idx, k);
@@ -343,6 +415,12 @@ const VariantEntry kVariants[] = {
     XMP_VARIANT(8, 29, 27),  // 6264 : 6144-bit
     XMP_VARIANT(8, 38, 27),  // 8208 : 8192-bit
     XMP_VARIANT(16, 19, 27), // 8208 : 8192-bit on sixteen lanes (one DPP row)
+    // small moduli, one lane per instance
+    XMP_VARIANT(1, 5, 28),   // 140 : 128-bit
+    XMP_VARIANT(1, 10, 28),  // 280 : 256-bit
+    // above 8192 bits: half a wave / a whole wave per instance, radix 2^26
+    XMP_VARIANT(32, 20, 26), // 16640 : 16384-bit
+    XMP_VARIANT(64, 20, 26), // 33280 : 32768-bit
 };
 #undef XMP_VARIANT
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
@@ -376,9 +454,13 @@ XMP_RUN(11, 8, 19, 27)
 XMP_RUN(12, 8, 29, 27)
 XMP_RUN(13, 8, 38, 27)
 XMP_RUN(14, 16, 19, 27)
+XMP_RUN(15, 1, 5, 28)
+XMP_RUN(16, 1, 10, 28)
 #undef XMP_RUN
-static_assert(kNumVariants == 15, "run_variant<> must list every compiled variant in kVariants order");
-static_assert(9 < kNumVariants, "variant 9 is the unbatched 2x55 layout");
+// Variants 9 (2x55, which spills as a called function), 17 and 18 (whole-wave
+// layouts above 8192 bits) are not part of the persistent kernel.
+static_assert(kNumVariants == 19, "run_variant<> must list every batched variant in kVariants order");
+constexpr bool batched_variant(int i) { return i >= 0 && i < kNumVariants && i != 9 && i != 17 && i != 18; }
 
 __global__ void __launch_bounds__(64, 2) powm_batch_kernel(BatchArgs ba) {
   extern __shared__ __attribute__((aligned(16))) uint32_t dyn_lds[];
@@ -405,6 +487,8 @@ __global__ void __launch_bounds__(64, 2) powm_batch_kernel(BatchArgs ba) {
       case 12: run_variant<12>(args, tile, dyn_lds); break;
       case 13: run_variant<13>(args, tile, dyn_lds); break;
       case 14: run_variant<14>(args, tile, dyn_lds); break;
+      case 15: run_variant<15>(args, tile, dyn_lds); break;
+      case 16: run_variant<16>(args, tile, dyn_lds); break;
       default: break;
     }
     __syncthreads();
@@ -426,19 +510,22 @@ int powm_find_variant(int lanes, int digits, int radix) {
   return -1;
 }
 
+// lazy accumulators: 2*ND products of < 2^(2D) plus a carry must fit in 64 bits
+constexpr uint32_t max_digits(int radix) { return radix == 28 ? 127u : radix == 27 ? 511u : radix == 26 ? 2047u : 0u; }
+static_assert(2.0 * 127 * 268435455.0 * 268435455.0 + 68719476736.0 < 18446744073709551616.0, "radix 28 bound");
+static_assert(2.0 * 511 * 134217727.0 * 134217727.0 + 137438953472.0 < 18446744073709551616.0, "radix 27 bound");
+static_assert(2.0 * 2047 * 67108863.0 * 67108863.0 + 274877906944.0 < 18446744073709551616.0, "radix 26 bound");
+
 bool variant_fits(const PowmVariant &v, uint32_t precision) {
   const uint32_t nd = (uint32_t)(v.lanes * v.digits);
-  // lazy accumulators: 2*ND products of < 2^(2D) plus a carry must fit in 64 bits
-  const uint32_t max_nd = v.radix == 28 ? 127u : 511u;
-  return nd <= max_nd && variant_rbits(v) >= precision + 2;
+  return nd <= max_digits(v.radix) && variant_rbits(v) >= precision + 2;
 }
 
 size_t powm_table_words(const PowmVariant &v, uint32_t count, uint32_t window_bits) {
   return (size_t)count * ((size_t)1 << window_bits) * (size_t)(v.lanes * v.digits);
 }
 
-// the 2-lane x 55-digit layout spills as a called function: not batched
-bool powm_variant_batchable(int variant) { return variant >= 0 && variant < kNumVariants && variant != 9; }
+bool powm_variant_batchable(int variant) { return batched_variant(variant); }
 
 size_t powm_variant_lds_bytes(int variant) {
   const PowmVariant &v = kVariants[variant].v;

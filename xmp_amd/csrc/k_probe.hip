@@ -18,11 +18,14 @@ __global__ void __launch_bounds__(64) mad_probe_kernel(uint64_t *out, uint32_t s
   const uint32_t a = threadIdx.x ^ seed, b = blockIdx.x + seed;
 #pragma unroll
   for (int i = 0; i < 8; i++) acc[i] = (uint64_t)i * seed;
+  // The multiply-adds are volatile asm: written in C, a * (b + i) is loop
+  // invariant and the compiler reduces each chain to a plain 64-bit add.
   for (int it = 0; it < kIters; it++) {
 #pragma unroll
-    for (int i = 0; i < 8; i++) acc[i] = (uint64_t)a * (uint32_t)(b + i) + acc[i];
-    // keep the chains dependent on the loop so they are not folded
-    __builtin_amdgcn_sched_barrier(0);
+    for (int i = 0; i < 8; i++) {
+      uint64_t carry;
+      asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(acc[i]), "=s"(carry) : "v"(a), "v"(b + i));
+    }
   }
   uint64_t r = 0;
 #pragma unroll

@@ -26,6 +26,7 @@ from xmp_amd import ext  # noqa: E402
 
 # (precision, exponent precision, batch sizes)
 PLAN = [
+    (128, 128, [4096, 65536, 262144]),
     (256, 256, [4096, 65536]),
     (512, 512, [4096, 65536, 262144]),
     (768, 768, [16384]),
@@ -36,6 +37,7 @@ PLAN = [
     (4096, 4096, [4096, 16384]),
     (6144, 6144, [4096]),
     (8192, 8192, [4096, 16384]),
+    (16384, 16384, [1024]),
 ]
 
 
@@ -56,7 +58,16 @@ def main():
                 print("%5d-bit x %6d: %2d measured in %5.1f s, best %dx%d radix %d window %d: %.0f powm/s" % (
                     bits, count, n, time.time() - t, best["lanes"], best["digits"], best["radix"], best["window"],
                     best["powm_per_s"]), flush=True)
-    table = {"device": "gfx950", "note": "measured by xmp_amd/tune.py (xmpTunerRun)", "entries": entries}
+    # keep the best window per (size, batch, layout): the runtime chooses among layouts
+    best = {}
+    for r in entries:
+        key = (r["precision"], r["exp_bits"], r["count"], r["lanes"], r["digits"], r["radix"])
+        if key not in best or r["powm_per_s"] > best[key]["powm_per_s"]:
+            best[key] = r
+    entries = sorted(best.values(), key=lambda r: (r["precision"], r["count"], -r["powm_per_s"]))
+    table = {"device": "gfx950",
+             "note": "measured by xmp_amd/tune.py (xmpTunerRun) on one MI355X; best window per layout and batch size",
+             "entries": entries}
     with open(args.out, "w") as f:
         json.dump(table, f, indent=1)
     print("wrote", args.out, len(entries), "entries")
