@@ -272,10 +272,14 @@ def main():
         add = lambda: lib.xmpIntegersAddAsync(h.h, S2.x, A2.x, B2.x, n2)  # noqa: E731
         sub = lambda: lib.xmpIntegersSubAsync(h.h, S2.x, A2.x, B2.x, n2)  # noqa: E731
         mul = lambda: lib.xmpIntegersMulAsync(h.h, P2.x, A2.x, B2.x, n2)  # noqa: E731
+        xor = lambda: lib.xmpIntegersXorAsync(h.h, S2.x, A2.x, B2.x, n2)  # noqa: E731
         add()
         hip.stream_sync(stream)
         add_ms = time_async(add, stream, 20)
         sub_ms = time_async(sub, stream, 20)
+        xor()
+        hip.stream_sync(stream)
+        xor_ms = time_async(xor, stream, 20)
         mul()
         hip.stream_sync(stream)
         mul_ms = time_async(mul, stream, 10)
@@ -283,10 +287,12 @@ def main():
                              "hbm_frac": round(768 * n2 / add_ms / 1e6 / HBM_PEAK_GBS, 4)}
         extra["sub_2048"] = {"pairs": n2, "ms": round(sub_ms, 4), "GB_per_s": round(768 * n2 / sub_ms / 1e6, 1),
                              "hbm_frac": round(768 * n2 / sub_ms / 1e6 / HBM_PEAK_GBS, 4)}
+        extra["xor_2048"] = {"pairs": n2, "ms": round(xor_ms, 4), "GB_per_s": round(768 * n2 / xor_ms / 1e6, 1),
+                             "hbm_frac": round(768 * n2 / xor_ms / 1e6 / HBM_PEAK_GBS, 4)}
         extra["mul_2048x2048"] = {"pairs": n2, "ms": round(mul_ms, 4), "mul_per_s": round(n2 / mul_ms * 1e3, 1),
                                   "GB_per_s": round(1024 * n2 / mul_ms / 1e6, 1),
                                   "TOP_per_s": round(2 * 4096 * n2 / mul_ms / 1e9, 3)}
-        log("add %.3f ms  sub %.3f ms  mul %.3f ms" % (add_ms, sub_ms, mul_ms))
+        log("add %.3f ms  sub %.3f ms  xor %.3f ms  mul %.3f ms" % (add_ms, sub_ms, xor_ms, mul_ms))
         del A2, B2, S2, P2
 
         # ---- div/mod: 65,536 x (4096-bit / 2048-bit, divisor top bit set)

@@ -44,6 +44,12 @@ static void (*p_fdiv_qr)(mpz_struct *, mpz_struct *, const mpz_struct *, const m
 static void (*p_mod)(mpz_struct *, const mpz_struct *, const mpz_struct *);
 static void (*p_mul_2exp)(mpz_struct *, const mpz_struct *, unsigned long);
 static void (*p_fdiv_r_2exp)(mpz_struct *, const mpz_struct *, unsigned long);
+static void (*p_tdiv_q_2exp)(mpz_struct *, const mpz_struct *, unsigned long);
+static void (*p_ior)(mpz_struct *, const mpz_struct *, const mpz_struct *);
+static void (*p_and)(mpz_struct *, const mpz_struct *, const mpz_struct *);
+static void (*p_xor)(mpz_struct *, const mpz_struct *, const mpz_struct *);
+static unsigned long (*p_popcount)(const mpz_struct *);
+static int (*p_cmp)(const mpz_struct *, const mpz_struct *);
 static const char *gmp_version_str;
 
 static int loaded = 0;
@@ -66,6 +72,12 @@ GO_API int go_load(void) {
   SYM(p_mod, "__gmpz_mod");
   SYM(p_mul_2exp, "__gmpz_mul_2exp");
   SYM(p_fdiv_r_2exp, "__gmpz_fdiv_r_2exp");
+  SYM(p_tdiv_q_2exp, "__gmpz_tdiv_q_2exp");
+  SYM(p_ior, "__gmpz_ior");
+  SYM(p_and, "__gmpz_and");
+  SYM(p_xor, "__gmpz_xor");
+  SYM(p_popcount, "__gmpz_popcount");
+  SYM(p_cmp, "__gmpz_cmp");
 #undef SYM
   const char **ver = (const char **)dlsym(h, "__gmp_version");
   gmp_version_str = ver ? *ver : "unknown";
@@ -86,7 +98,8 @@ static void exp_trunc(uint32_t *x, int n, const mpz_struct *z) {
   free(tmp);
 }
 
-enum { GO_ADD = 0, GO_SUB = 1, GO_MUL = 2, GO_DIV = 3, GO_MOD = 4, GO_POWM = 5, GO_POWM_SEC = 6 };
+enum { GO_ADD = 0, GO_SUB = 1, GO_MUL = 2, GO_DIV = 3, GO_MOD = 4, GO_POWM = 5, GO_POWM_SEC = 6,
+       GO_IOR = 7, GO_AND = 8, GO_XOR = 9, GO_SHF = 10, GO_POPC = 11, GO_CMP = 12 };
 
 typedef struct {
   int op;
@@ -104,7 +117,7 @@ static void *worker(void *arg) {
   p_init(za); p_init(zb); p_init(zc); p_init(zr); p_init(zs);
   for (uint32_t i = j->lo; i < j->hi; i++) {
     imp(za, j->a + (size_t)(i % j->ac) * j->an, j->an);
-    if (j->b) imp(zb, j->b + (size_t)(i % j->bc) * j->bn, j->bn);
+    if (j->b && j->op != GO_SHF) imp(zb, j->b + (size_t)(i % j->bc) * j->bn, j->bn);
     if (j->c) imp(zc, j->c + (size_t)(i % j->cc) * j->cn, j->cn);
     switch (j->op) {
       case GO_ADD: p_add(zr, za, zb); break;
@@ -117,6 +130,24 @@ static void *worker(void *arg) {
       case GO_DIV: case GO_MOD: p_fdiv_qr(zr, zs, za, zb); break;
       case GO_POWM: p_powm(zr, za, zb, zc); break;       /* a^b mod c */
       case GO_POWM_SEC: p_powm_sec(zr, za, zb, zc); break;
+      /* bitwise ops as the reference tests compute them (unit_tests/test.cu:2516-2595
+       * mpz_and/ior/xor, :1366-1372 mpz_mul_2exp / mpz_tdiv_q_2exp, :2091 mpz_cmp,
+       * :2191 mpz_popcount); shift and the popc / cmp results are one int32 limb */
+      case GO_IOR: p_ior(zr, za, zb); break;
+      case GO_AND: p_and(zr, za, zb); break;
+      case GO_XOR: p_xor(zr, za, zb); break;
+      case GO_SHF: {
+        int32_t sh = (int32_t)j->b[i % j->bc];
+        if (sh >= 0) p_mul_2exp(zr, za, (unsigned long)sh);
+        else p_tdiv_q_2exp(zr, za, (unsigned long)(-(int64_t)sh));
+        break;
+      }
+      case GO_POPC: j->out[(size_t)i * j->on] = (uint32_t)p_popcount(za); continue;
+      case GO_CMP: {
+        int r = p_cmp(za, zb);
+        j->out[(size_t)i * j->on] = (uint32_t)(int32_t)(r > 0 ? 1 : r < 0 ? -1 : 0);
+        continue;
+      }
     }
     if (j->out) exp_trunc(j->out + (size_t)i * j->on, j->on, j->op == GO_MOD ? zs : zr);
     if (j->out2) exp_trunc(j->out2 + (size_t)i * j->on2, j->on2, zs);

@@ -327,6 +327,67 @@ XO_API void xo_transform(void *output, const void *input, uint32_t count, uint32
   }
 }
 
+/* ---- bitwise operations (reference src/logic.cu, kernels in
+ * src/include/implementations.h:766-1067).  Inputs are zero-extended to the
+ * output length; outputs longer than both inputs are zero-filled (ior, and,
+ * xor) or one-filled (not). ---- */
+
+/* c = a | b (implementations.h:850-894). */
+XO_API void xo_ior(limb_t *c, int cn, const limb_t *a, int an, const limb_t *b, int bn) {
+  for (int i = 0; i < cn; i++) c[i] = get_limb(a, an, i) | get_limb(b, bn, i);
+}
+
+/* c = a & b (implementations.h:896-936). */
+XO_API void xo_and(limb_t *c, int cn, const limb_t *a, int an, const limb_t *b, int bn) {
+  for (int i = 0; i < cn; i++) c[i] = get_limb(a, an, i) & get_limb(b, bn, i);
+}
+
+/* c = a ^ b (implementations.h:938-982). */
+XO_API void xo_xor(limb_t *c, int cn, const limb_t *a, int an, const limb_t *b, int bn) {
+  for (int i = 0; i < cn; i++) c[i] = get_limb(a, an, i) ^ get_limb(b, bn, i);
+}
+
+/* c = ~a over cn limbs, limbs above a's length all ones (implementations.h:984-1018,
+ * unit_tests/test.cu:1480-1488). */
+XO_API void xo_not(limb_t *c, int cn, const limb_t *a, int an) {
+  for (int i = 0; i < cn; i++) c[i] = i < an ? ~a[i] : ~(limb_t)0;
+}
+
+/* c = a * 2^shift mod 2^(32 cn) for shift >= 0, floor(a / 2^-shift) for
+ * shift < 0 (implementations.h:1020-1067; the reference test builds the
+ * same value with mpz_mul_2exp / mpz_tdiv_q_2exp, unit_tests/test.cu:1366-1372).
+ * Output limb i holds bits [32 i - shift, 32 i - shift + 32) of a. */
+XO_API void xo_shf(limb_t *c, int cn, const limb_t *a, int an, int32_t shift) {
+  for (int i = 0; i < cn; i++) {
+    limb_t v = 0;
+    for (int k = 0; k < 32; k++) {
+      int64_t bit = 32 * (int64_t)i + k - (int64_t)shift;
+      if (bit >= 0 && bit < 32 * (int64_t)an && ((a[bit >> 5] >> (bit & 31)) & 1u)) v |= (limb_t)1 << k;
+    }
+    c[i] = v;
+  }
+}
+
+/* number of one bits (implementations.h:821-848). */
+XO_API uint32_t xo_popc(const limb_t *a, int an) {
+  uint32_t n = 0;
+  for (int i = 0; i < an; i++)
+    for (limb_t x = a[i]; x; x &= x - 1) n++;
+  return n;
+}
+
+/* -1, 0, +1 as a <, ==, > b, with the shorter operand zero-extended
+ * (implementations.h:766-819; the reference test compares sgn of mpz_cmp,
+ * unit_tests/test.cu:2091-2098). */
+XO_API int32_t xo_cmp(const limb_t *a, int an, const limb_t *b, int bn) {
+  int n = an > bn ? an : bn;
+  for (int i = n - 1; i >= 0; i--) {
+    limb_t x = get_limb(a, an, i), y = get_limb(b, bn, i);
+    if (x != y) return x > y ? 1 : -1;
+  }
+  return 0;
+}
+
 /* ---- batch drivers over compact arrays ([count][limbs]) with the reference's
  * broadcast rule: input instance i reads element i % array_count
  * (src/include/implementations.h:58,68 and docs/xmp.docx para 15-17). ---- */

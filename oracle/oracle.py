@@ -19,7 +19,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
-GO_ADD, GO_SUB, GO_MUL, GO_DIV, GO_MOD, GO_POWM, GO_POWM_SEC = range(7)
+(GO_ADD, GO_SUB, GO_MUL, GO_DIV, GO_MOD, GO_POWM, GO_POWM_SEC,
+ GO_IOR, GO_AND, GO_XOR, GO_SHF, GO_POPC, GO_CMP) = range(13)
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 
@@ -31,9 +32,17 @@ def lib():
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
         L = ctypes.CDLL(path)
-        for name in ("xo_add", "xo_sub", "xo_mul"):
+        for name in ("xo_add", "xo_sub", "xo_mul", "xo_ior", "xo_and", "xo_xor"):
             getattr(L, name).argtypes = [_u32p, ctypes.c_int, _u32p, ctypes.c_int, _u32p, ctypes.c_int]
             getattr(L, name).restype = None
+        L.xo_not.argtypes = [_u32p, ctypes.c_int, _u32p, ctypes.c_int]
+        L.xo_not.restype = None
+        L.xo_shf.argtypes = [_u32p, ctypes.c_int, _u32p, ctypes.c_int, ctypes.c_int32]
+        L.xo_shf.restype = None
+        L.xo_popc.argtypes = [_u32p, ctypes.c_int]
+        L.xo_popc.restype = ctypes.c_uint32
+        L.xo_cmp.argtypes = [_u32p, ctypes.c_int, _u32p, ctypes.c_int]
+        L.xo_cmp.restype = ctypes.c_int32
         L.xo_divmod.argtypes = [_u32p, ctypes.c_int, _u32p, ctypes.c_int, _u32p, ctypes.c_int, _u32p, ctypes.c_int]
         L.xo_divmod.restype = ctypes.c_int
         L.xo_np0.argtypes = [ctypes.c_uint32]
@@ -94,6 +103,49 @@ def mul(a, b, out_limbs):
     return out
 
 
+def _bitwise(name, a, b, out_limbs):
+    a, b = _c(a), _c(b)
+    out = np.zeros(out_limbs, np.uint32)
+    getattr(lib(), name)(_p(out), out_limbs, _p(a), a.size, _p(b), b.size)
+    return out
+
+
+def ior(a, b, out_limbs):
+    return _bitwise("xo_ior", a, b, out_limbs)
+
+
+def and_(a, b, out_limbs):
+    return _bitwise("xo_and", a, b, out_limbs)
+
+
+def xor(a, b, out_limbs):
+    return _bitwise("xo_xor", a, b, out_limbs)
+
+
+def not_(a, out_limbs):
+    a = _c(a)
+    out = np.zeros(out_limbs, np.uint32)
+    lib().xo_not(_p(out), out_limbs, _p(a), a.size)
+    return out
+
+
+def shf(a, shift, out_limbs):
+    a = _c(a)
+    out = np.zeros(out_limbs, np.uint32)
+    lib().xo_shf(_p(out), out_limbs, _p(a), a.size, int(shift))
+    return out
+
+
+def popc(a):
+    a = _c(a)
+    return int(lib().xo_popc(_p(a), a.size))
+
+
+def cmp(a, b):
+    a, b = _c(a), _c(b)
+    return int(lib().xo_cmp(_p(a), a.size, _p(b), b.size))
+
+
 def divmod_(a, b, q_limbs, r_limbs):
     a, b = _c(a), _c(b)
     q = np.zeros(max(q_limbs, 1), np.uint32)
@@ -152,8 +204,11 @@ def gmp_version():
 
 def gmp_batch(op, count, out_limbs, a, b=None, c=None, out2_limbs=0, threads=1):
     """Run a GMP op over `count` instances (input row i % rows).  Returns
-    (out, out2, seconds).  op: GO_ADD/SUB/MUL/DIV/MOD/POWM/POWM_SEC; for powm
-    a=base, b=exp, c=mod.  Results are truncated to out_limbs limbs."""
+    (out, out2, seconds).  op: GO_ADD/SUB/MUL/DIV/MOD/POWM/POWM_SEC and the
+    bitwise GO_IOR/AND/XOR/SHF/POPC/CMP; for powm a=base, b=exp, c=mod; for
+    GO_SHF b holds one int32 shift per row (viewed as uint32); GO_POPC and
+    GO_CMP write one word per instance (cmp as int32 -1/0/1).  Results are
+    truncated to out_limbs limbs."""
     a = _c(a)
     b = _c(b) if b is not None else None
     c = _c(c) if c is not None else None

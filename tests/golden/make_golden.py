@@ -12,7 +12,10 @@ cases:
     samples/simple_rsa/rsa.cc:126-130;
   * import/export repacking for every order, endian and word size
     (test.cu:296-579), computed from the word-transform definition of
-    src/xmp.cu:461-489.
+    src/xmp.cu:461-489;
+  * bitwise ops as the reference tests define them (test.cu:1366-1372
+    shifts, :1480-1488 not, :2091 cmp, :2191 popcount, :2516-2595
+    and/ior/xor).
 
 Run: python tests/golden/make_golden.py   (deterministic; rewrites the file)
 """
@@ -127,6 +130,30 @@ def main():
                     cases.append({"size": size, "order": order, "endian": endian, "nails": nails, "count": count,
                                   "words": words, "data": data.hex(), "limbs": out})
     g["import"] = cases
+
+    # bitwise ops (src/logic.cu): inputs zero-extended to the output width,
+    # not fills with ones, shift positive = left (truncated), negative = right;
+    # popcount and sign comparison
+    r2 = random.Random(20261203)
+    cases = []
+    for an, bn, on in [(1, 1, 1), (2, 1, 2), (4, 4, 4), (3, 5, 6), (8, 2, 8), (16, 16, 16)]:
+        for k in range(3):
+            a, b = rnd(r2, 32 * an), rnd(r2, 32 * bn)
+            if k == 0:
+                a = (1 << (32 * an)) - 1
+            mod = 1 << (32 * on)
+            shifts = sorted({0, 1, -1, 31, -31, 32, -32, 33, -33, 32 * an, -32 * an, 32 * on + 5, -(32 * an) - 5,
+                             r2.randrange(-64 * an, 64 * an)})
+            cases.append({"an": an, "bn": bn, "on": on, "a": hexs(a), "b": hexs(b),
+                          "ior": hexs(a | b), "and": hexs(a & b), "xor": hexs(a ^ b),
+                          "not": hexs((mod - 1) ^ a),
+                          "shf": [[sh, hexs(((a << sh) % mod) if sh >= 0 else (a >> -sh))] for sh in shifts],
+                          "popc": bin(a).count("1"), "cmp": (a > b) - (a < b)})
+    # equal values of different lengths compare equal; a leading-zero longer operand compares by value
+    cases.append({"an": 4, "bn": 2, "on": 4, "a": hexs(0x1234), "b": hexs(0x1234), "ior": hexs(0x1234),
+                  "and": hexs(0x1234), "xor": "0", "not": hexs(((1 << 128) - 1) ^ 0x1234),
+                  "shf": [[0, hexs(0x1234)]], "popc": bin(0x1234).count("1"), "cmp": 0})
+    g["bitwise"] = cases
 
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(g, f, indent=0, sort_keys=True)

@@ -187,6 +187,89 @@ def test_planted_mistakes_are_caught():
     assert caught
 
 
+# ---------------------------------------------------------------- bitwise ops
+def test_golden_bitwise():
+    for c in GOLDEN["bitwise"]:
+        a, b, on = L(H(c["a"]), c["an"]), L(H(c["b"]), c["bn"]), c["on"]
+        assert O.to_int(O.ior(a, b, on)) == H(c["ior"])
+        assert O.to_int(O.and_(a, b, on)) == H(c["and"])
+        assert O.to_int(O.xor(a, b, on)) == H(c["xor"])
+        assert O.to_int(O.not_(a, on)) == H(c["not"])
+        for sh, v in c["shf"]:
+            assert O.to_int(O.shf(a, sh, on)) == H(v), (c, sh)
+        assert O.popc(a) == c["popc"]
+        assert O.cmp(a, b) == c["cmp"] and O.cmp(b, a) == -c["cmp"]
+
+
+def test_bitwise_oracle_vs_gmp_and_int():
+    if not O.gmp_available():
+        pytest.skip("libgmp.so.10 not present")
+    r = random.Random(11)
+    n = 48
+    for an, bn, on in ((1, 1, 1), (3, 2, 4), (8, 8, 8), (16, 5, 16), (64, 64, 64)):
+        a = inputs.uniform(600 + an, n, an)
+        b = inputs.uniform(700 + bn, n, bn)
+        a[0, :] = b[0, :min(an, bn)].tolist() + [0] * max(0, an - bn)   # an equal pair
+        for op, fn in ((O.GO_IOR, O.ior), (O.GO_AND, O.and_), (O.GO_XOR, O.xor)):
+            want = gmp(op, n, on, a, b)
+            for i in range(n):
+                assert np.array_equal(fn(a[i], b[i], on), want[i]), (op, i)
+        shifts = np.array([r.randrange(-40 * an, 40 * an) for _ in range(n)], np.int32)
+        shifts[:4] = [0, 32, -32, 32 * an]
+        want = gmp(O.GO_SHF, n, on, a, shifts.view(np.uint32).reshape(-1, 1))
+        for i in range(n):
+            got = O.shf(a[i], int(shifts[i]), on)
+            assert np.array_equal(got, want[i]), int(shifts[i])
+            x, sh = O.to_int(a[i]), int(shifts[i])
+            assert O.to_int(got) == (((x << sh) % (1 << (32 * on))) if sh >= 0 else x >> -sh)
+        pc = gmp(O.GO_POPC, n, 1, a)
+        cm = gmp(O.GO_CMP, n, 1, a, b).view(np.int32)
+        for i in range(n):
+            assert O.popc(a[i]) == int(pc[i, 0]) == bin(O.to_int(a[i])).count("1")
+            assert O.cmp(a[i], b[i]) == int(cm[i, 0])
+            x, y = O.to_int(a[i]), O.to_int(b[i])
+            assert O.cmp(a[i], b[i]) == (x > y) - (x < y)
+
+
+def test_bitwise_closed_forms():
+    r = random.Random(5)
+    for _ in range(40):
+        n = r.randrange(1, 9)
+        x, y = r.getrandbits(32 * n), r.getrandbits(32 * n)
+        a, b = L(x, n), L(y, n)
+        # De Morgan and self-inverse identities
+        assert np.array_equal(O.not_(O.and_(a, b, n), n), O.ior(O.not_(a, n), O.not_(b, n), n))
+        assert O.to_int(O.xor(a, a, n)) == 0 and np.array_equal(O.not_(O.not_(a, n), n), a)
+        # a left shift undone by a right shift when nothing falls off the top
+        s = r.randrange(0, 32 * n)
+        wide = O.shf(a, s, 2 * n)
+        assert np.array_equal(O.shf(wide, -s, n), a)
+        # popcount(a) + popcount(~a) = width
+        assert O.popc(a) + O.popc(O.not_(a, n)) == 32 * n
+    assert O.cmp(L(5, 3), L(5, 1)) == 0 and O.cmp(L(1 << 64, 3), L((1 << 64) - 1, 2)) == 1
+
+
+def test_bitwise_planted_mistakes_are_caught():
+    """Plausible slips in a copy of the bitwise formulas disagree with the fixtures."""
+    cases = GOLDEN["bitwise"]
+
+    def not_zero_fill(a, an, on):          # limbs above a's length filled with 0, not 1
+        return ((1 << (32 * an)) - 1) ^ a
+
+    def shf_sign_flipped(a, sh, on):        # positive shift taken as a right shift
+        return (a >> sh) if sh >= 0 else ((a << -sh) % (1 << (32 * on)))
+
+    def shf_limb_off_by_one(a, sh, on):     # whole-limb shifts moved one limb too far
+        t = sh + 32 if sh != 0 and sh % 32 == 0 else sh
+        return ((a << t) % (1 << (32 * on))) if t >= 0 else (a >> -t)
+
+    assert any(not_zero_fill(H(c["a"]), c["an"], c["on"]) != H(c["not"]) for c in cases)
+    for bad in (shf_sign_flipped, shf_limb_off_by_one):
+        assert any(bad(H(c["a"]), sh, c["on"]) != H(v) for c in cases for sh, v in c["shf"]), bad.__name__
+    assert any(bin(H(c["a"]) & 0xFFFFFFFF).count("1") != c["popc"] for c in cases)   # low limb only
+    assert any(-c["cmp"] != c["cmp"] for c in cases)                                   # sign convention
+
+
 def test_input_stream_is_portable():
     """The seeded generator is a fixed bit stream (PCG64 raw output)."""
     x = inputs.uniform(12345, 2, 4)

@@ -84,6 +84,14 @@ def _declare(L):
         "xmpIntegersDivModAsync": [_vp, _vp, _vp, _vp, _vp, _u32],
         "xmpIntegersPowm": [_vp, _vp, _vp, _vp, _vp, _u32],
         "xmpIntegersPowmAsync": [_vp, _vp, _vp, _vp, _vp, _u32],
+        "xmpIntegersNot": [_vp, _vp, _vp, _u32],
+        "xmpIntegersNotAsync": [_vp, _vp, _vp, _u32],
+        "xmpIntegersShf": [_vp, _vp, _vp, _vp, _u32, _u32],
+        "xmpIntegersShfAsync": [_vp, _vp, _vp, _vp, _u32, _u32],
+        "xmpIntegersPopc": [_vp, _vp, _vp, _u32],
+        "xmpIntegersPopcAsync": [_vp, _vp, _vp, _u32],
+        "xmpIntegersCmp": [_vp, _vp, _vp, _vp, _u32],
+        "xmpIntegersCmpAsync": [_vp, _vp, _vp, _vp, _u32],
     }
     for n in ("Add", "Sub", "Mul", "Div", "Mod", "Ior", "And", "Xor"):
         sig["xmpIntegers" + n] = [_vp, _vp, _vp, _vp, _u32]
@@ -209,6 +217,44 @@ class Handle:
         c = count if count is not None else out.count
         f = lib().xmpIntegersPowmAsync if asynchronous else lib().xmpIntegersPowm
         check(f(self.h, out.x, base.x, exp.x, mod.x, c), "xmpIntegersPowm")
+
+    # ---- bitwise (reference xmp.h:206-226)
+    def ior(self, out, a, b, count=None):
+        self._bin("xmpIntegersIor", out, a, b, count if count is not None else out.count)
+
+    def and_(self, out, a, b, count=None):
+        self._bin("xmpIntegersAnd", out, a, b, count if count is not None else out.count)
+
+    def xor(self, out, a, b, count=None):
+        self._bin("xmpIntegersXor", out, a, b, count if count is not None else out.count)
+
+    def not_(self, out, a, count=None):
+        c = count if count is not None else out.count
+        check(lib().xmpIntegersNot(self.h, out.x, a.x, c), "xmpIntegersNot")
+
+    def shf(self, out, a, shifts, count=None):
+        """shifts: int32 array (host numpy) or a device pointer (int); instance i uses shifts[i % len]."""
+        c = count if count is not None else out.count
+        if isinstance(shifts, int):
+            raise TypeError("pass a device pointer with shf_device")
+        sh = np.ascontiguousarray(shifts, dtype=np.int32)
+        check(lib().xmpIntegersShf(self.h, out.x, a.x, sh.ctypes.data, sh.size, c), "xmpIntegersShf")
+
+    def shf_device(self, out, a, shift_ptr, shift_count, count=None):
+        c = count if count is not None else out.count
+        check(lib().xmpIntegersShf(self.h, out.x, a.x, shift_ptr, shift_count, c), "xmpIntegersShf")
+
+    def popc(self, a, count=None):
+        c = count if count is not None else a.count
+        res = np.zeros(c, np.uint32)
+        check(lib().xmpIntegersPopc(self.h, res.ctypes.data, a.x, c), "xmpIntegersPopc")
+        return res
+
+    def cmp(self, a, b, count=None):
+        c = count if count is not None else max(a.count, b.count)
+        res = np.zeros(c, np.int32)
+        check(lib().xmpIntegersCmp(self.h, res.ctypes.data, a.x, b.x, c), "xmpIntegersCmp")
+        return res
 
 
 class Integers:

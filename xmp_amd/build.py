@@ -25,9 +25,11 @@ SOURCES = [
     "batch.cpp",
     "serve.cpp",
     "ext.cpp",
+    "logic.cpp",
     "k_xfer.hip",
     "k_arith.hip",
     "k_div.hip",
+    "k_logic.hip",
     "k_powm.hip",
     "k_probe.hip",
 ]

@@ -56,7 +56,7 @@ xmpError_t set_necessary_tmp_out(xmpHandle_t h, size_t bytes) {
 }
 
 // true when `p` is device-accessible memory (device or managed allocation)
-static bool is_device_pointer(const void *p) {
+bool is_device_pointer(const void *p) {
   hipPointerAttribute_t attr;
   hipError_t e = hipPointerGetAttributes(&attr, p);
   if (e != hipSuccess) {

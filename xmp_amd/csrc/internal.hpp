@@ -93,6 +93,10 @@ inline bool hip_ok() { return hipPeekAtLastError() == hipSuccess; }
 #define XMP_CHECK_HIP() \
   if (!xmp::hip_ok()) return xmpErrorCuda;
 
+// true when `p` is device-accessible memory; other pointers are treated as
+// host memory (reference cudaPointerGetAttributes test, src/xmp.cu:522-531)
+bool is_device_pointer(const void *p);
+
 // grow-only scratch owned by the handle (reference src/xmp.cu:105-129)
 xmpError_t set_necessary_scratch(xmpHandle_t h, size_t bytes);
 // grow-only temporary output for in-place ops (reference src/xmp.cu:131-148)

@@ -42,6 +42,16 @@ hipError_t launch_mul(OutOperand p, Operand a, Operand b, uint32_t count, hipStr
 // q and/or r may have data == nullptr to skip that output.
 hipError_t launch_divmod(OutOperand q, OutOperand r, Operand a, Operand b, uint32_t count, hipStream_t stream);
 
+// ---- bitwise (k_logic.hip), reference src/logic.cu ----
+enum : int { kBitIor = 0, kBitAnd = 1, kBitXor = 2, kBitNot = 3 };
+// c = a op b (zero-extended to c.len; not ignores b and fills with ones)
+hipError_t launch_bitwise(int op, OutOperand c, Operand a, Operand b, uint32_t count, hipStream_t stream);
+// c = a shifted by shift[i % shift_count] bits (positive left, negative right); shift on the device
+hipError_t launch_shift(OutOperand c, Operand a, const int32_t *shift, uint32_t shift_count, uint32_t count,
+                        hipStream_t stream);
+hipError_t launch_popc(uint32_t *out, Operand a, uint32_t count, hipStream_t stream);
+hipError_t launch_cmp(int32_t *out, Operand a, Operand b, uint32_t count, hipStream_t stream);
+
 // ---- Montgomery powm (k_powm.hip) ----
 struct PowmPlan;  // powm.hpp
 

@@ -285,43 +285,4 @@ xmpError_t XMPAPI xmpIntegersPowm(xmpHandle_t handle, xmpIntegers_t out, const x
   return sync(handle, xmpIntegersPowmAsync(handle, out, base, exp, mod, count));
 }
 
-// ---- bitwise operations: outside this library's scope (reference logic.cu) ----
-xmpError_t XMPAPI xmpIntegersShf(xmpHandle_t, xmpIntegers_t, const xmpIntegers_t, const int32_t *, const uint32_t,
-                                 uint32_t) {
-  return xmpErrorUnsupported;
-}
-xmpError_t XMPAPI xmpIntegersShfAsync(xmpHandle_t, xmpIntegers_t, const xmpIntegers_t, const int32_t *,
-                                      const uint32_t, uint32_t) {
-  return xmpErrorUnsupported;
-}
-#define XMP_UNSUPPORTED_BIN(name)                                                                       \
-  xmpError_t XMPAPI name(xmpHandle_t, xmpIntegers_t, const xmpIntegers_t, const xmpIntegers_t, uint32_t) { \
-    return xmpErrorUnsupported;                                                                         \
-  }
-XMP_UNSUPPORTED_BIN(xmpIntegersIor)
-XMP_UNSUPPORTED_BIN(xmpIntegersIorAsync)
-XMP_UNSUPPORTED_BIN(xmpIntegersAnd)
-XMP_UNSUPPORTED_BIN(xmpIntegersAndAsync)
-XMP_UNSUPPORTED_BIN(xmpIntegersXor)
-XMP_UNSUPPORTED_BIN(xmpIntegersXorAsync)
-#undef XMP_UNSUPPORTED_BIN
-xmpError_t XMPAPI xmpIntegersNot(xmpHandle_t, xmpIntegers_t, const xmpIntegers_t, uint32_t) {
-  return xmpErrorUnsupported;
-}
-xmpError_t XMPAPI xmpIntegersNotAsync(xmpHandle_t, xmpIntegers_t, const xmpIntegers_t, uint32_t) {
-  return xmpErrorUnsupported;
-}
-xmpError_t XMPAPI xmpIntegersPopc(xmpHandle_t, uint32_t *, const xmpIntegers_t, uint32_t) {
-  return xmpErrorUnsupported;
-}
-xmpError_t XMPAPI xmpIntegersPopcAsync(xmpHandle_t, uint32_t *, const xmpIntegers_t, uint32_t) {
-  return xmpErrorUnsupported;
-}
-xmpError_t XMPAPI xmpIntegersCmp(xmpHandle_t, int32_t *, const xmpIntegers_t, const xmpIntegers_t, uint32_t) {
-  return xmpErrorUnsupported;
-}
-xmpError_t XMPAPI xmpIntegersCmpAsync(xmpHandle_t, int32_t *, const xmpIntegers_t, const xmpIntegers_t, uint32_t) {
-  return xmpErrorUnsupported;
-}
-
 }  // extern "C"

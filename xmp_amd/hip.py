@@ -35,6 +35,7 @@ def hip():
         H.hipMalloc.argtypes = [ctypes.POINTER(_vp), ctypes.c_size_t]
         H.hipFree.argtypes = [_vp]
         H.hipMemset.argtypes = [_vp, ctypes.c_int, ctypes.c_size_t]
+        H.hipMemcpy.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_int]
     return _hip
 
 
@@ -92,3 +93,10 @@ def malloc(nbytes, fill=None):
 
 def free(ptr):
     _ok(hip().hipFree(_vp(ptr)), "hipFree")
+
+
+def memcpy_htod(ptr, arr):
+    """Synchronous copy of a contiguous numpy array to device memory at ptr."""
+    import numpy as np
+    a = np.ascontiguousarray(arr)
+    _ok(hip().hipMemcpy(_vp(ptr), _vp(a.ctypes.data), a.nbytes, 1), "hipMemcpy")   # 1 = hipMemcpyHostToDevice
