@@ -172,6 +172,13 @@ def run_only(args):
         ctx = Montgomery(h, M, 2048, 65536)
         fn = lambda: ctx.powm_async(out, B, E, 65536)  # noqa: E731
         what = "powm_kernel 65,536 x 2048-bit (algorithmic bytes: 4 x 256 B in/out per instance)"
+    elif args.only == "mul":
+        n2 = 1 << 20
+        a2, b2 = inputs.ordered_pair(4242, n2, 64)
+        A2, B2 = h.from_array(a2), h.from_array(b2)
+        P2 = h.integers(4096, n2)
+        fn = lambda: lib.xmpIntegersMulAsync(h.h, P2.x, A2.x, B2.x, n2)  # noqa: E731
+        what = "mul_lanes_kernel<2,37,28> 1,048,576 x 2048x2048 -> 4096-bit (4096 canonical MACs and 1024 B per product)"
     elif args.only == "add":
         n2 = 1 << 20
         a2, b2 = inputs.ordered_pair(4242, n2, 64)
@@ -200,7 +207,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["powm", "add", "copy", "mixed"],
+    ap.add_argument("--only", choices=["powm", "add", "mul", "copy", "mixed"],
                     help="run just one kernel (for counter profiling): the 2048-bit powm kernel with resident "
                          "constants, the 2048-bit add, or a plain 256 MiB import copy for counter calibration")
     args = ap.parse_args()
@@ -327,7 +334,7 @@ def main():
         log("mixed batch: batched %.1f ms, plain %.1f ms (x%.2f)" % (
             mixed["batched_ms"], mixed["plain_ms"], mixed["batched_over_plain"]))
         # ---- serving: 1024 resident keys, Poisson arrivals
-        serving, serve_sample = run_serving(h, 1024, 40000, 99)
+        serving, serve_sample = run_serving(h, 1024, 200000, 99)
         extra["serving"] = serving
         log("serving: %s" % json.dumps(serving))
 

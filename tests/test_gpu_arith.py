@@ -110,7 +110,11 @@ def test_add_in_place(handle):
 
 @pytest.mark.parametrize("bits_a,bits_b,bits_p", [(32, 32, 64), (128, 128, 256), (512, 512, 1024),
                                                   (1024, 1024, 2048), (2048, 2048, 4096), (2048, 2048, 2048),
-                                                  (2048, 1024, 3072), (96, 2048, 2144), (4096, 4096, 8192)])
+                                                  (2048, 1024, 3072), (96, 2048, 2144), (4096, 4096, 8192),
+                                                  (1536, 1536, 3072), (1536, 512, 1600), (2048, 2048, 4160),
+                                                  (2048, 1024, 2080), (4096, 2048, 4096), (8192, 8192, 16384),
+                                                  (8192, 4096, 8192), (12288, 64, 12352), (16384, 16384, 32768),
+                                                  (16384, 16384, 16384), (32, 2048, 2080)])
 def test_mul(handle, bits_a, bits_b, bits_p):
     n = 500
     a, b = _pair(51, n, bits_a // 32, bits_b // 32)
@@ -118,6 +122,51 @@ def test_mul(handle, bits_a, bits_b, bits_p):
     P = handle.integers(bits_p, n)
     handle.mul(P, A, B)
     assert_rows_equal(P.get(), gmp(O.GO_MUL, n, bits_p // 32, a, b), "mul")
+
+
+@pytest.mark.parametrize("bits", [1536, 2048, 4096, 8192, 16384])
+def test_mul_all_ones_and_indices(handle, bits):
+    """Operands of all ones (every accumulator at its bound, the longest carry
+    chains) and shuffled index arrays on slots 0..2 with a broadcast multiplier."""
+    n, limbs = 130, bits // 32
+    ones = np.full((n, limbs), 0xFFFFFFFF, np.uint32)
+    P = handle.integers(2 * bits, n)
+    A = handle.from_array(ones)
+    handle.mul(P, A, A)
+    want = O.from_int(((1 << bits) - 1) ** 2, 2 * limbs)
+    assert all(np.array_equal(r, want) for r in P.get())
+    a = inputs.uniform(53 + bits, n, limbs)
+    b = inputs.uniform(54 + bits, 3, limbs)
+    rng = np.random.default_rng(bits)
+    ia, ip = rng.permutation(n).astype(np.uint32), rng.permutation(n).astype(np.uint32)
+    A, B = handle.from_array(a), handle.from_array(b)
+    handle.set_indices(0, ip)
+    handle.set_indices(1, ia)
+    try:
+        handle.mul(P, A, B)
+    finally:
+        handle.clear_indices()
+    rows = gmp(O.GO_MUL, n, 2 * limbs, a[ia], b[np.arange(n) % 3])
+    want = np.zeros_like(rows)
+    want[ip] = rows
+    assert_rows_equal(P.get(), want, "indexed mul")
+
+
+def test_mul_full_size_sample(handle):
+    """1,048,576 x 2048-bit products (the benchmark shape): a sample of rows
+    against GMP, and a * 1 == a on every row."""
+    n = 1 << 20
+    a, b = inputs.ordered_pair(57, n, 64)
+    A, B = handle.from_array(a), handle.from_array(b)
+    P = handle.integers(4096, n)
+    handle.mul(P, A, B)
+    sample = np.arange(0, n, 1009)
+    assert_rows_equal(P.get()[sample], gmp(O.GO_MUL, sample.size, 128, a[sample], b[sample]), "mul sample")
+    one = np.zeros((1, 64), np.uint32)
+    one[0, 0] = 1
+    handle.mul(P, A, handle.from_array(one))
+    got = P.get()
+    assert np.array_equal(got[:, :64], a) and not got[:, 64:].any()
 
 
 def test_mul_closed_form_and_square(handle):

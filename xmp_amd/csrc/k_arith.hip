@@ -9,12 +9,14 @@
 // registers (v_add_co_u32/v_addc_co_u32).
 //
 // mul (reference src/mul.cu:140-303, digitmp/mul.h:24-55): p = a*b mod
-// 2^P(p).  The longer operand is split into 28-bit digits held in registers
-// and the product is formed row by row in 64-bit "lazy" accumulators: each
-// partial product is one v_mad_u64_u32 and carries are resolved once per
-// retired digit, because up to 256 products of 56 bits fit in 64 bits.
+// 2^P(p).  The longer operand is split into radix-2^28 (2^27 above 4096 bits)
+// digits held in the registers of one lane or a group of lanes, and the
+// product is formed row by row in 64-bit "lazy" accumulators: each partial
+// product is one v_mad_u64_u32 and carries are resolved once per retired
+// digit, because up to 255 products of 56 bits fit in 64 bits.
 #include "devmp.hpp"
 #include "kernels.hpp"
+#include "lanes.hpp"
 
 namespace xmp {
 
@@ -124,85 +126,159 @@ __global__ void __launch_bounds__(256) addsub_general_kernel(OutOperand s, Opera
 }
 
 // ------------------------------------------------------------------ mul
-constexpr int kDigit = 28;
-constexpr uint32_t kDigitMask = (1u << kDigit) - 1;
 
-// Load NL 32-bit limbs of one instance and split them into ND 28-bit digits.
-template <int NL, int ND>
-__device__ __forceinline__ void load_digits(uint32_t (&d)[ND], const Operand &o, uint32_t inst) {
-  uint32_t l[NL + 1];
-#pragma unroll
-  for (int k = 0; k < NL; k++) l[k] = ld_limb(o, inst, k);
-  l[NL] = 0;
-#pragma unroll
-  for (int k = 0; k < ND; k++) {
-    const int bit = kDigit * k;
-    const int w = bit >> 5, sh = bit & 31;
-    uint32_t lo = w < NL + 1 ? l[w] : 0u;
-    uint32_t hi = w + 1 < NL + 1 ? l[w + 1] : 0u;
-    d[k] = (sh ? ((lo >> sh) | (hi << (32 - sh))) : lo) & kDigitMask;
-  }
-}
-
-// Row scanning (operand scanning) over radix-2^28 digits: the multiplicand's
-// NDA digits and NDA lazy 64-bit accumulators stay in registers; the
-// multiplier is streamed from HBM one digit per row through a bit buffer, and
-// each row retires one finished product digit, so the accumulator window
-// never grows.  Output digits are repacked into 32-bit limbs on the fly and
-// truncated to p.len limbs.
-template <int NLA>
-__global__ void __launch_bounds__(64, (NLA >= 64 ? 1 : 2)) mul_rows_kernel(OutOperand p, Operand a, Operand b, uint32_t count) {
-  constexpr int NDA = (32 * NLA + kDigit - 1) / kDigit;
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= count) return;
-  const uint32_t ai = in_index(a, t), bi = in_index(b, t), pi = out_index(p, t);
-  uint32_t da[NDA];
-  load_digits<NLA, NDA>(da, a, ai);
-  uint64_t acc[NDA];
-#pragma unroll
-  for (int k = 0; k < NDA; k++) acc[k] = 0;
-  uint64_t ibuf = 0;   // multiplier bits not yet consumed
-  uint32_t inb = 0, next = 0;
-  uint64_t obuf = 0;   // product bits not yet stored
+// Row scanning (operand scanning) over radix-2^D digits: the multiplicand's
+// L*K digits and the accumulator window are spread over a group of L lanes
+// (lane l holds digits l*K .. l*K+K-1; L = 1 up to 1024 bits), so the
+// registers per lane stay near 4K and three waves fit per SIMD.  The
+// multiplier's digits are staged in LDS first and read four at a time (an LDS
+// broadcast within the group).  Each row multiplies the window by one
+// multiplier digit and retires the lowest digit, which lane 0 packs into
+// output limbs; the window then moves down one digit with the same
+// split-carry DPP shift as the Montgomery product.  Rows stop once the
+// retired digits cover the output (truncated products skip the rest); else
+// the remaining window is normalised and each lane writes the limbs that
+// start in its digits.
+// Bound: every accumulator receives at most ND_b products < 2^(2D) plus a
+// carry < 2^(64-D) and a D-bit digit (ND_b <= 255 at D = 28, <= 1023 at D = 27).
+template <int L, int K, int D>
+struct MulRow {
+  static constexpr uint32_t MASK = (1u << D) - 1;
+  const Group<L> &g;
+  uint64_t obuf = 0;   // product bits not yet stored (meaningful on lane 0)
   uint32_t onb = 0, limb = 0;
-  const uint32_t ndb = (32u * b.len + kDigit - 1) / kDigit;
-  auto emit = [&](uint32_t digit) {
-    obuf |= (uint64_t)digit << onb;
-    onb += kDigit;
+  __device__ __forceinline__ explicit MulRow(const Group<L> &grp) : g(grp) {}
+  // one row: window += a * d, then retire (and return) the lowest digit
+  __device__ __forceinline__ uint32_t retire(uint64_t (&t)[K], const uint32_t (&da)[K], uint32_t d) const {
+    uint64_t f = t[0] + (uint64_t)da[0] * d;
+    const uint32_t lo = (uint32_t)f & MASK;
+    const uint32_t up = g.from_next(lo);
+#pragma unroll
+    for (int j = 1; j < K; j++) t[j - 1] = t[j] + (uint64_t)da[j] * d;
+    t[K - 1] = up;
+    t[0] += f >> D;
+    return lo;
+  }
+  // one row with the retired digit packed into the bit buffer
+  __device__ __forceinline__ void row(uint64_t (&t)[K], const uint32_t (&da)[K], uint32_t d, const OutOperand &p,
+                                      uint32_t pi, bool store) {
+    const uint32_t lo = retire(t, da, d);
+    obuf |= (uint64_t)lo << onb;
+    onb += D;
     if (onb >= 32) {
-      if (limb < p.len) p.data[(size_t)limb * p.stride + pi] = (uint32_t)obuf;
+      if (store && limb < p.len) p.data[(size_t)limb * p.stride + pi] = (uint32_t)obuf;
       limb++;
       obuf >>= 32;
       onb -= 32;
     }
-  };
-  for (uint32_t i = 0; i < ndb; i++) {
-    if (inb < (uint32_t)kDigit) {
-      ibuf |= (uint64_t)ld_limb(b, bi, next++) << inb;
-      inb += 32;
+  }
+};
+
+template <int L, int K, int D>
+__global__ void __launch_bounds__(64, 2) mul_lanes_kernel(OutOperand p, Operand a, Operand b, uint32_t count) {
+  constexpr int ND = L * K;
+  constexpr int G = 64 / L;
+  constexpr int NDP = (ND + 3) / 4 * 4;   // the multiplier's digits; 16-byte rows
+  __shared__ __attribute__((aligned(16))) uint32_t stage[G * NDP];
+  const int lane = threadIdx.x;
+  const int grp = lane / L;
+  const Group<L> g(lane);
+  const int l = g.l;
+  const uint32_t inst = blockIdx.x * G + grp;
+  const bool active = inst < count;
+  const uint32_t i = active ? inst : count - 1;   // inactive groups compute a valid instance and store nothing
+  const uint32_t ai = in_index(a, i), bi = in_index(b, i);
+  const uint32_t pi = active ? out_index(p, i) : 0u;
+  const bool store = active && l == 0;
+  uint32_t *st = stage + grp * NDP;
+
+  uint32_t da[K];
+  uint64_t t[K];
+  {
+    uint32_t db[K];   // the multiplier's digits (b.len <= a.len), staged for the rows
+    load_lane_digits<K, D>(db, b.data, b.stride, bi, b.len, (uint32_t)(l * K));
+#pragma unroll
+    for (int k = 0; k < K; k++) st[l * K + k] = db[k];
+  }
+  load_lane_digits<K, D>(da, a.data, a.stride, ai, a.len, (uint32_t)(l * K));
+#pragma unroll
+  for (int k = 0; k < K; k++) t[k] = 0;
+  __syncthreads();
+  const uint32_t ndb = (32u * b.len + D - 1) / D;
+  const uint32_t nout = (32u * p.len + D - 1) / D;   // digits that reach the output
+  const uint32_t rows = ndb < nout ? ndb : nout;
+  MulRow<L, K, D> m(g);
+  uint32_t r = 0;
+  for (; r + 4 <= rows; r += 4) {
+    const uint4 bb = *reinterpret_cast<const uint4 *>(st + r);
+    m.row(t, da, bb.x, p, pi, store);
+    m.row(t, da, bb.y, p, pi, store);
+    m.row(t, da, bb.z, p, pi, store);
+    m.row(t, da, bb.w, p, pi, store);
+  }
+  for (; r < rows; r++) m.row(t, da, st[r], p, pi, store);
+  if (rows < ndb) {
+    // truncated: the retired digits already cover every output bit
+    return;
+  }
+  // The rest of the product: the window's ND digits, from bit D*rows upward.
+  // Lane l's digits start at bit S_l = D*(rows + l*K); it writes the limbs
+  // whose first bit lies in [S_l, S_l+1) (lane 0 from limb `limb`, after its
+  // pending bits), reading the next lane's two lowest digits for the last one.
+  uint32_t rd[K];
+  normalize_group<L, K, D>(rd, t, g);
+  const uint32_t n0 = g.from_next(rd[0]), n1 = g.from_next(rd[1]);
+  const uint32_t s_here = (uint32_t)D * (rows + (uint32_t)(l * K));
+  const uint32_t s_next = s_here + (uint32_t)(D * K);
+  const uint32_t wend = (l == L - 1) ? p.len : min(p.len, (s_next + 31) / 32);
+  uint64_t buf;
+  uint32_t nb, w;
+  auto emit = [&]() {
+    if (nb >= 32) {
+      if (active && w < wend) p.data[(size_t)w * p.stride + pi] = (uint32_t)buf;
+      w++;
+      buf >>= 32;
+      nb -= 32;
     }
-    const uint32_t d = (uint32_t)ibuf & kDigitMask;
-    ibuf >>= kDigit;
-    inb -= kDigit;
-    uint64_t f = acc[0] + (uint64_t)da[0] * d;
+  };
+  if (l == 0) {
+    // after the pending bits of limb `limb`
+    buf = m.obuf;
+    nb = m.onb;
+    w = m.limb;
+    buf |= (uint64_t)rd[0] << nb;
+    nb += D;
+    emit();
+    buf |= (uint64_t)rd[1] << nb;
+    nb += D;
+    emit();
+  } else {
+    // from the first limb boundary at or above S_l
+    w = (s_here + 31) / 32;
+    const uint32_t skip = 32 * w - s_here;
+    buf = (((uint64_t)rd[1] << D) | rd[0]) >> skip;
+    nb = 2 * D - skip;
+    emit();
+  }
 #pragma unroll
-    for (int j = 1; j < NDA; j++) acc[j - 1] = acc[j] + (uint64_t)da[j] * d;
-    acc[NDA - 1] = 0;
-    acc[0] += f >> kDigit;
-    emit((uint32_t)f & kDigitMask);
+  for (int k = 2; k < K + 2; k++) {
+    const uint32_t x = k < K ? rd[k < K ? k : 0] : (k == K ? n0 : n1);
+    buf |= (uint64_t)x << nb;
+    nb += D;
+    emit();
   }
-  uint64_t c = 0;
-#pragma unroll
-  for (int k = 0; k < NDA; k++) {
-    uint64_t x = acc[k] + c;
-    emit((uint32_t)x & kDigitMask);
-    c = x >> kDigit;
+  if (l == L - 1 && active) {
+    // past the product: the partial limb, then zeros
+    if (nb > 0 && w < wend) p.data[(size_t)w * p.stride + pi] = (uint32_t)buf;
+    for (w += (nb > 0 ? 1u : 0u); w < p.len; w++) p.data[(size_t)w * p.stride + pi] = 0u;
   }
-  if (onb > 0) {
-    if (limb < p.len) p.data[(size_t)limb * p.stride + pi] = (uint32_t)obuf;
-    limb++;
-  }
-  for (uint32_t k = limb; k < p.len; k++) p.data[(size_t)k * p.stride + pi] = 0u;
+}
+
+template <int L, int K, int D>
+hipError_t launch_mul_lanes(OutOperand p, Operand a, Operand b, uint32_t count, hipStream_t stream) {
+  constexpr uint32_t G = 64 / L;
+  hipLaunchKernelGGL((mul_lanes_kernel<L, K, D>), dim3(div_round_up(count, G)), dim3(64), 0, stream, p, a, b, count);
+  return hipGetLastError();
 }
 
 // Generic fallback for any sizes: schoolbook with 32-bit limbs, the row
@@ -235,11 +311,6 @@ __global__ void __launch_bounds__(256) mul_general_kernel(OutOperand p, Operand 
   }
 }
 
-template <int NLA>
-hipError_t launch_mul_rows(OutOperand p, Operand a, Operand b, uint32_t count, hipStream_t stream) {
-  hipLaunchKernelGGL((mul_rows_kernel<NLA>), dim3(div_round_up(count, 64)), dim3(64), 0, stream, p, a, b, count);
-  return hipGetLastError();
-}
 
 }  // namespace
 
@@ -264,13 +335,18 @@ hipError_t launch_sub(OutOperand d, Operand a, Operand b, uint32_t count, hipStr
 }
 
 hipError_t launch_mul(OutOperand p, Operand a, Operand b, uint32_t count, hipStream_t stream) {
-  // register-resident row kernels: the longer operand (a) in registers, any multiplier length
-  if (a.len <= 4) return launch_mul_rows<4>(p, a, b, count, stream);
-  if (a.len <= 8) return launch_mul_rows<8>(p, a, b, count, stream);
-  if (a.len <= 16) return launch_mul_rows<16>(p, a, b, count, stream);
-  if (a.len <= 32) return launch_mul_rows<32>(p, a, b, count, stream);
-  if (a.len <= 48) return launch_mul_rows<48>(p, a, b, count, stream);
-  if (a.len <= 64) return launch_mul_rows<64>(p, a, b, count, stream);
+  // register-resident row kernels, the longer operand (a) as the multiplicand:
+  // one lane per instance up to 1024 bits, then lane groups; b.len <= a.len
+  // keeps the multiplier within the accumulator bound
+  if (a.len <= 4) return launch_mul_lanes<1, 5, 28>(p, a, b, count, stream);      // 140 bits
+  if (a.len <= 8) return launch_mul_lanes<1, 10, 28>(p, a, b, count, stream);     // 280 bits
+  if (a.len <= 16) return launch_mul_lanes<1, 19, 28>(p, a, b, count, stream);    // 532 bits
+  if (a.len <= 32) return launch_mul_lanes<1, 37, 28>(p, a, b, count, stream);   // 1036 bits
+  if (a.len <= 48) return launch_mul_lanes<2, 28, 28>(p, a, b, count, stream);    // 56 digits >= 1536 bits
+  if (a.len <= 64) return launch_mul_lanes<2, 37, 28>(p, a, b, count, stream);    // 74 digits >= 2048 bits
+  if (a.len <= 128) return launch_mul_lanes<4, 37, 28>(p, a, b, count, stream);   // 148 digits >= 4096 bits
+  if (a.len <= 256) return launch_mul_lanes<8, 38, 27>(p, a, b, count, stream);   // 304 digits >= 8192 bits
+  if (a.len <= 512) return launch_mul_lanes<16, 38, 27>(p, a, b, count, stream);  // 608 digits >= 16384 bits
   hipLaunchKernelGGL(mul_general_kernel, dim3(div_round_up(count, 256)), dim3(256), 0, stream, p, a, b, count);
   return hipGetLastError();
 }

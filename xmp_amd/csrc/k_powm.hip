@@ -36,80 +36,14 @@
 #include <utility>
 
 #include "devmp.hpp"
+#include "lanes.hpp"
 #include "powm.hpp"
 
 namespace xmp {
 
 namespace {
 
-// Cross-lane moves inside an instance's group of L lanes (groups are aligned:
-// L <= 16 stays inside a 16-lane DPP row), done with DPP moves on the VALU
-// instead of LDS permutes:
-//   bcast0    -- the group's first lane's value to every lane of the group
-//   from_next -- lane l receives lane l+1's value, the group's last lane 0
-//   from_prev -- lane l receives lane l-1's value, the group's first lane 0
-// DPP controls: quad_perm [0,0,2,2] = 0xA0, [0,0,0,0] = 0x00, row_shl:1 =
-// 0x101, row_shr:1 = 0x111, row_shr:4 = 0x114, row_newbcast:0 = 0x150 (lane 0
-// of each row to the whole row), row_bcast:15 = 0x142 (lane 15 of each row to
-// the next row), wave_shl:1 = 0x130 and wave_shr:1 = 0x138 (whole-wave shifts,
-// for groups that span rows).  A 64-lane group reads its first lane with
-// readfirstlane, which leaves the quotient digit in a scalar register.
-template <int L>
-struct Group {
-  static_assert(L == 1 || L == 2 || L == 4 || L == 8 || L == 16 || L == 32 || L == 64, "group size");
-  static constexpr int SHL1 = L <= 16 ? 0x101 : 0x130;
-  static constexpr int SHR1 = L <= 16 ? 0x111 : 0x138;
-  int l;   // lane within the instance group
-  __device__ __forceinline__ explicit Group(int lane) : l(lane & (L - 1)) {}
-  __device__ __forceinline__ uint32_t bcast0(uint32_t x) const {
-    if constexpr (L == 1) {
-      return x;
-    } else if constexpr (L == 2) {
-      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xA0, 0xF, 0xF, false);
-    } else if constexpr (L == 4) {
-      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x00, 0xF, 0xF, false);
-    } else if constexpr (L == 8) {
-      uint32_t y = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x00, 0xF, 0xF, false);
-      uint32_t z = (uint32_t)__builtin_amdgcn_mov_dpp((int)y, 0x114, 0xF, 0xF, true);
-      return (l & 4) ? z : y;
-    } else if constexpr (L == 16) {
-      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x150, 0xF, 0xF, false);
-    } else if constexpr (L == 32) {
-      // every row takes its lane 0; rows 1 and 3 then take lane 15 of rows 0 and 2
-      uint32_t y = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x150, 0xF, 0xF, false);
-      return (uint32_t)__builtin_amdgcn_update_dpp((int)y, (int)y, 0x142, 0xA, 0xF, false);
-    } else {
-      return (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
-    }
-  }
-  __device__ __forceinline__ uint32_t from_next(uint32_t x) const {
-    if constexpr (L == 1) {
-      return 0;
-    } else {
-      uint32_t v = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, SHL1, 0xF, 0xF, true);
-      if constexpr (L == 16 || L == 64) return v;   // the row's / wave's last lane already reads 0
-      return l == L - 1 ? 0u : v;
-    }
-  }
-  __device__ __forceinline__ uint64_t from_prev(uint64_t x) const {
-    if constexpr (L == 1) {
-      return 0;
-    } else {
-      uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)x, SHR1, 0xF, 0xF, true);
-      uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(x >> 32), SHR1, 0xF, 0xF, true);
-      uint64_t v = ((uint64_t)hi << 32) | lo;
-      if constexpr (L == 16 || L == 64) return v;   // the row's / wave's first lane already reads 0
-      return l == 0 ? 0ull : v;
-    }
-  }
-  __device__ __forceinline__ bool any(bool p) const {
-    if (L == 1) return p;
-    uint32_t v = p ? 1u : 0u;
-#pragma unroll
-    for (int o = 1; o < L; o <<= 1) v |= (uint32_t)__shfl_xor((int)v, o, L);
-    return v != 0;
-  }
-};
+
 
 template <int L, int K, int D>
 struct Mont {
@@ -157,36 +91,8 @@ struct Mont {
     normalize(r, t);
   }
 
-  // Resolve the lazy sums into D-bit digits.  Each lane first carries through
-  // its own K digits; the lane carries (< 2^(64-D)) then ripple up the group
-  // one lane per round.  After the first round a lane passes a carry on only
-  // if its digits were all ones, so the rounds stop as soon as no lane of the
-  // wave holds a carry (almost always after one round).
   __device__ __forceinline__ void normalize(uint32_t (&r)[K], const uint64_t (&t)[K]) const {
-    uint64_t c = 0;
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      uint64_t x = t[k] + c;
-      r[k] = (uint32_t)x & MASK;
-      c = x >> D;
-    }
-    if constexpr (L > 1) {
-      for (int round = 1; round < L; round++) {
-        c = g.from_prev(c);
-        if (round > 1 && __builtin_amdgcn_ballot_w64(c != 0) == 0) break;
-        // c < 2^(64-D): past the first digit the carry fits in 32 bits
-        uint64_t x = (uint64_t)r[0] + c;
-        r[0] = (uint32_t)x & MASK;
-        uint32_t c32 = (uint32_t)(x >> D);
-#pragma unroll
-        for (int k = 1; k < K; k++) {
-          uint32_t y = r[k] + c32;
-          r[k] = y & MASK;
-          c32 = y >> D;
-        }
-        c = c32;
-      }
-    }
+    normalize_group<L, K, D>(r, t, g);
   }
 
   // r = a^2 / R (mod m) for one-lane instances: the CIOS steps with the
@@ -232,16 +138,6 @@ struct Mont {
   }
 };
 
-// digit k of an integer given by 32-bit limbs (interleaved) as radix-2^D
-template <int D>
-__device__ __forceinline__ uint32_t limb_digit(const uint32_t *data, uint32_t stride, uint32_t inst, uint32_t len,
-                                               uint32_t digit) {
-  uint32_t bit = (uint32_t)D * digit;
-  uint32_t w = bit >> 5, s = bit & 31;
-  uint32_t lo = w < len ? data[(size_t)w * stride + inst] : 0u;
-  uint32_t hi = w + 1 < len ? data[(size_t)(w + 1) * stride + inst] : 0u;
-  return funnel(hi, lo, s) & ((1u << D) - 1);
-}
 
 __device__ __forceinline__ uint32_t exp_bits(const Operand &e, uint32_t inst, uint32_t pos, uint32_t len) {
   uint32_t w = pos >> 5, s = pos & 31;
@@ -276,8 +172,8 @@ __device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, u
 #pragma unroll
   for (int k = 0; k < K; k++) {
     uint32_t d = (uint32_t)(l * K + k);
-    mont.m[k] = limb_digit<D>(args.mod.data, args.mod.stride, mi, args.mod.len, d);
-    b[l * K + k] = limb_digit<D>(args.r2, args.r2_stride, mi, args.mod.len, d);
+    mont.m[k] = limb_digit_guarded<D>(args.mod.data, args.mod.stride, mi, args.mod.len, d);
+    b[l * K + k] = limb_digit_guarded<D>(args.r2, args.r2_stride, mi, args.mod.len, d);
   }
   {
     // np0 = -m^-1 mod 2^D (Newton, as reference regmp/np0.h:24-33)
@@ -303,7 +199,7 @@ __device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, u
   // slot 1 = base * R mod m
 #pragma unroll
   for (int k = 0; k < K; k++)
-    a[k] = limb_digit<D>(args.base.data, args.base.stride, bi, args.base.len, (uint32_t)(l * K + k));
+    a[k] = limb_digit_guarded<D>(args.base.data, args.base.stride, bi, args.base.len, (uint32_t)(l * K + k));
   mont.mul(a, a, b);
   __syncthreads();
   // slots 2..T-1: slot[e] = slot[e-1] * slot[1]
