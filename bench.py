@@ -62,6 +62,12 @@ def time_async(fn, stream, reps):
     return e0.elapsed_ms(e1) / reps
 
 
+def lib_powm_async(h, out, base, exp, mod, count):
+    rc = xmp_amd.lib().xmpIntegersPowmAsync(h.h, out.x, base.x, exp.x, mod.x, count)
+    if rc != 0:
+        raise xmp_amd.XmpError(rc, "xmpIntegersPowmAsync")
+
+
 def cpu_cores():
     try:
         n = len(os.sched_getaffinity(0))
@@ -270,6 +276,21 @@ def main():
 
     extra = {}
     if not args.quick:
+        # ---- the same 65,536 x 2048-bit work as one persistent graph launch vs the plain call
+        OUT2 = h.integers(BITS, N)
+        batch = Batch(h, [OUT2], [B], [E], [M], [N])
+        batch.launch_async()
+        hip.stream_sync(stream)
+        b_ms = time_async(batch.launch_async, stream, max(2, args.steps // 2))
+        p_ms = time_async(lambda: lib_powm_async(h, OUT2, B, E, M, N), stream, max(2, args.steps // 2))
+        same = bool(np.array_equal(OUT2.get(), got))
+        batch.close()
+        extra["batched_2048"] = {"instances": N, "batched_ms": round(b_ms, 3), "plain_ms": round(p_ms, 3),
+                                 "batched_powm_per_s": round(N / b_ms * 1e3, 1),
+                                 "batched_over_plain": round(p_ms / b_ms, 3), "same_results": same}
+        log("2048-bit batched %.3f ms vs plain %.3f ms" % (b_ms, p_ms))
+        del OUT2
+
         # ---- elementwise add/sub/mul on 1,048,576 pairs of 2048-bit integers
         n2 = 1 << 20
         a2, b2 = inputs.ordered_pair(4242, n2, 64)

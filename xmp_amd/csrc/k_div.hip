@@ -84,6 +84,59 @@ struct DivOut {
   uint32_t r_len, r_stride, r_inst;
 };
 
+// One quotient digit j of Knuth's algorithm D: estimate q from the window's
+// top limbs (2-by-1 division by the top divisor limb with its reciprocal,
+// refined twice with the second limb), subtract q*V from the window Win with
+// two hardware borrow chains (low and high product words), add V back in the
+// rare case the estimate was one too large, and write the difference shifted
+// up one limb into Wout with the next dividend limb below it.
+template <int NB>
+__device__ __forceinline__ void digit_step(const uint32_t (&Win)[NB + 1], uint32_t (&Wout)[NB + 1], uint32_t next,
+                                           const uint32_t (&v)[NB], uint32_t dinv, const DivOut &o, int32_t j) {
+  uint32_t qhat, rhat;
+  bool rhat_ok = true;
+  if (Win[NB] >= v[NB - 1]) {
+    qhat = 0xFFFFFFFFu;
+    uint64_t rr = (uint64_t)Win[NB - 1] + v[NB - 1];
+    rhat = (uint32_t)rr;
+    rhat_ok = (rr >> 32) == 0;
+  } else {
+    qhat = div_2by1(Win[NB], Win[NB - 1], v[NB - 1], dinv, rhat);
+  }
+  if (NB > 1) {
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+      if (rhat_ok && (uint64_t)qhat * v[NB - 2] > (((uint64_t)rhat << 32) | Win[NB > 1 ? NB - 2 : 0])) {
+        qhat--;
+        uint64_t rr = (uint64_t)rhat + v[NB - 1];
+        rhat = (uint32_t)rr;
+        rhat_ok = (rr >> 32) == 0;
+      }
+    }
+  }
+  // Win - qhat * V: limb k loses lo(qhat*v[k]) and hi(qhat*v[k-1])
+  uint32_t b1 = 0, b2 = 0, ph = 0;
+#pragma unroll
+  for (int k = 0; k < NB; k++) {
+    const uint64_t p = (uint64_t)qhat * v[k];
+    const uint32_t x = __builtin_subc(Win[k], (uint32_t)p, b1, &b1);
+    Wout[k + 1] = __builtin_subc(x, ph, b2, &b2);
+    ph = (uint32_t)(p >> 32);
+  }
+  {
+    const uint32_t x = __builtin_subc(Win[NB], ph, b1, &b1);
+    (void)__builtin_subc(x, 0u, b2, &b2);
+  }
+  if (b1 | b2) {   // estimate one too large: add V back (the carry out cancels the top limb)
+    qhat--;
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < NB; k++) Wout[k + 1] = __builtin_addc(Wout[k + 1], v[k], c, &c);
+  }
+  Wout[0] = next;
+  if (o.q && (uint32_t)j < o.q_len) o.q[(size_t)j * o.q_stride + o.q_inst] = qhat;
+}
+
 // Divide the streamed dividend by the NB-limb divisor loaded from (bdata,
 // bstride, binst, blen).  Returns false on division by zero.
 template <int NB, class Dividend>
@@ -129,7 +182,6 @@ __device__ __forceinline__ void divide_reg(const Dividend &A, const uint32_t *bd
   };
   const int32_t lu = (int32_t)la + sl + 1;   // limbs of U
   const int32_t nq = lu - NB;                // quotient limbs computed
-  uint32_t W[NB + 1];
   if (nq <= 0) {
     // |a| < 2^(32*NB) * 2^-shift: quotient 0, remainder a
     if (o.q)
@@ -138,57 +190,33 @@ __device__ __forceinline__ void divide_reg(const Dividend &A, const uint32_t *bd
       for (uint32_t k = 0; k < o.r_len; k++) o.r[(size_t)k * o.r_stride + o.r_inst] = A((int32_t)k);
     return;
   }
-  // window = U[nq-1 .. nq-1+NB]
+  // window = U[nq-1 .. nq-1+NB]; two windows in turn: each digit's
+  // multiply-subtract writes the next (shifted) window, so no limb is moved
+  uint32_t Wa[NB + 1], Wb[NB + 1];
 #pragma unroll
-  for (int k = 0; k <= NB; k++) W[k] = U(nq - 1 + k);
-  uint32_t next = U(nq - 2);
-  for (int32_t j = nq - 1; j >= 0; j--) {
-    // quotient digit estimate from the top limbs
-    uint32_t qhat, rhat;
-    bool rhat_ok = true;
-    if (W[NB] >= v[NB - 1]) {
-      qhat = 0xFFFFFFFFu;
-      uint64_t rr = (uint64_t)W[NB - 1] + v[NB - 1];
-      rhat = (uint32_t)rr;
-      rhat_ok = (rr >> 32) == 0;
-    } else {
-      qhat = div_2by1(W[NB], W[NB - 1], v[NB - 1], dinv, rhat);
-    }
-    if (NB > 1) {
-#pragma unroll
-      for (int it = 0; it < 2; it++) {
-        if (rhat_ok && (uint64_t)qhat * v[NB - 2] > (((uint64_t)rhat << 32) | W[NB - 2])) {
-          qhat--;
-          uint64_t rr = (uint64_t)rhat + v[NB - 1];
-          rhat = (uint32_t)rr;
-          rhat_ok = (rr >> 32) == 0;
-        }
-      }
-    }
-    // W -= qhat * V
-    uint32_t carry = 0, borrow = 0;
-#pragma unroll
-    for (int k = 0; k < NB; k++) {
-      uint64_t p = (uint64_t)qhat * v[k] + carry;
-      carry = (uint32_t)(p >> 32);
-      W[k] = subb(W[k], (uint32_t)p, borrow, borrow);
-    }
-    W[NB] = subb(W[NB], carry, borrow, borrow);
-    if (borrow) {   // estimate one too large: add back
-      qhat--;
-      uint32_t c = 0;
-#pragma unroll
-      for (int k = 0; k < NB; k++) W[k] = addc(W[k], v[k], c, c);
-      W[NB] += c;
-    }
-    if (o.q && (uint32_t)j < o.q_len) o.q[(size_t)j * o.q_stride + o.q_inst] = qhat;
-    if (j > 0) {
-#pragma unroll
-      for (int k = NB; k > 0; k--) W[k] = W[k - 1];
-      W[0] = next;
-      next = U(j - 2);
-    }
+  for (int k = 0; k <= NB; k++) Wa[k] = U(nq - 1 + k);
+  // the dividend limbs that enter the window, read four digits ahead
+  uint32_t n0 = U(nq - 2), n1 = U(nq - 3), n2 = U(nq - 4), n3 = U(nq - 5);
+  int32_t j = nq - 1;
+  for (; j >= 1; j -= 2) {
+    digit_step<NB>(Wa, Wb, n0, v, dinv, o, j);
+    digit_step<NB>(Wb, Wa, n1, v, dinv, o, j - 1);
+    // digits j-2 and j-3 take U(j-3) and U(j-4); U(j-5), U(j-6) are read for the pair after
+    n0 = n2;
+    n1 = n3;
+    n2 = U(j - 5);
+    n3 = U(j - 6);
   }
+  if (j == 0) {
+    digit_step<NB>(Wa, Wb, 0u, v, dinv, o, 0);
+#pragma unroll
+    for (int k = 0; k <= NB; k++) Wa[k] = Wb[k];
+  }
+  // the remainder is Wa[1 .. NB] (the last step shifted it up one limb)
+  uint32_t W[NB + 1];
+#pragma unroll
+  for (int k = 0; k < NB; k++) W[k] = Wa[k + 1];
+  W[NB] = 0;
   if (o.q)
     for (uint32_t k = (uint32_t)nq; k < o.q_len; k++) o.q[(size_t)k * o.q_stride + o.q_inst] = 0u;
   if (o.r) {

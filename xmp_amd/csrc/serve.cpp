@@ -154,7 +154,13 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
   if ((e = hipMemcpyAsync(sl.d_idx, sl.h_idx, sizeof(uint32_t) * kIdxWords * cap, hipMemcpyHostToDevice, st)) != hipSuccess)
     return e;
   const size_t words = (size_t)cap * n32;
-  if ((e = hipMemcpyAsync(sl.d_in, sl.h_in, sizeof(uint32_t) * 3 * words, hipMemcpyHostToDevice, st)) != hipSuccess)
+  // only the rows this batch uses: bases and exponents of the na powm requests, c of the nb CRT requests
+  if (na && ((e = hipMemcpyAsync(sl.d_in, sl.h_in, sizeof(uint32_t) * na * n32, hipMemcpyHostToDevice, st)) ||
+             (e = hipMemcpyAsync(sl.d_in + words, sl.h_in + words, sizeof(uint32_t) * na * n32,
+                                 hipMemcpyHostToDevice, st))))
+    return e;
+  if (nb && (e = hipMemcpyAsync(sl.d_in + 2 * words, sl.h_in + 2 * words, sizeof(uint32_t) * nb * n32,
+                                hipMemcpyHostToDevice, st)))
     return e;
   const uint8_t *in_base = (const uint8_t *)sl.d_in, *in_exp = (const uint8_t *)(sl.d_in + words),
                 *in_c = (const uint8_t *)(sl.d_in + 2 * words);
@@ -210,7 +216,10 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
                            nullptr, 0, st)) != hipSuccess)
       return e;
   }
-  if ((e = hipMemcpyAsync(sl.h_out, sl.d_out, sizeof(uint32_t) * 2 * words, hipMemcpyDeviceToHost, st)) != hipSuccess)
+  if (na && (e = hipMemcpyAsync(sl.h_out, sl.d_out, sizeof(uint32_t) * na * n32, hipMemcpyDeviceToHost, st)))
+    return e;
+  if (nb && (e = hipMemcpyAsync(sl.h_out + words, sl.d_out + words, sizeof(uint32_t) * nb * n32,
+                                hipMemcpyDeviceToHost, st)))
     return e;
   (void)h32;
   return hipEventRecord(sl.done, st);
