@@ -31,12 +31,12 @@ HBM_PEAK_GBS = 8000.0
 # at 2.4 GHz = the published 78.6 TFLOP/s FP64 vector rate of the same datapath;
 # a multiply-add counts as 2 integer operations
 VALU_PEAK_TOPS = 78.6
-# HBM bytes per launch of powm_kernel<2,37,28> with window 7 on 65,536 x
-# 2048-bit instances, from --pmc runs of `bench.py --only powm`: reads are
+# HBM bytes per launch of powm_kernel<2,37,28> on 65,536 x 2048-bit instances
+# by window width, from --pmc runs of `bench.py --only powm`: reads are
 # 32/64/128 x TCC_EA0_RDREQ_{32B,64B,128B} (exact on a copy kernel of known
 # size), writes are WRITE_SIZE (exact on the same copy)
-POWM_TRAFFIC_BYTES = 7917544640 + 2808747328
-POWM_TRAFFIC_LAYOUT = (2, 37, 28, 7)
+POWM_TRAFFIC_BYTES = {(2, 37, 28, 6): 9121351872 + 1393325632,
+                      (2, 37, 28, 7): 7917544640 + 2808747328}
 
 
 def canonical_powm_macs(bits, exp_bits):
@@ -269,7 +269,7 @@ def main():
     from xmp_amd.ext import probe_mad_rate
     roofline["measured_mad_per_clk_per_cu"] = {"2_waves": round(probe_mad_rate(h, 2), 2),
                                                "8_waves": round(probe_mad_rate(h, 8), 2), "full_rate": 64}
-    roofline["traffic"] = POWM_TRAFFIC_BYTES if (lanes, digits, radix, wbits) == POWM_TRAFFIC_LAYOUT else None
+    roofline["traffic"] = POWM_TRAFFIC_BYTES.get((lanes, digits, radix, wbits))
     log("powm kernel %.3f ms -> %.2f TOP/s (%.1f%% of %.1f)" % (kern_ms, achieved, 100 * achieved / VALU_PEAK_TOPS,
                                                             VALU_PEAK_TOPS))
     ctx.close()

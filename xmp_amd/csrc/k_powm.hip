@@ -45,6 +45,17 @@ namespace {
 
 
 
+// Every powm workgroup is a single wave64, and the LDS operations of one wave
+// complete in program order, so the staging through LDS needs no workgroup
+// barrier: a wavefront-scope fence keeps the compiler from moving LDS
+// accesses across the hand-over, and costs no instruction.  (Inside the
+// persistent kernel's non-inlined tile functions the compiler cannot see the
+// workgroup size and would otherwise emit a barrier with a full wait.)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 template <int L, int K, int D>
 struct Mont {
   static constexpr int ND = L * K;
@@ -192,7 +203,7 @@ __device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, u
   // slot 0 = R mod m = 1 * (R^2 mod m) / R
 #pragma unroll
   for (int k = 0; k < K; k++) a[k] = (l == 0 && k == 0) ? 1u : 0u;
-  __syncthreads();
+  wave_sync();
   mont.mul(a, a, b);
 #pragma unroll
   for (int k = 0; k < K; k++) *tab_at(0, k) = a[k];
@@ -201,20 +212,20 @@ __device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, u
   for (int k = 0; k < K; k++)
     a[k] = limb_digit_guarded<D>(args.base.data, args.base.stride, bi, args.base.len, (uint32_t)(l * K + k));
   mont.mul(a, a, b);
-  __syncthreads();
+  wave_sync();
   // slots 2..T-1: slot[e] = slot[e-1] * slot[1]
 #pragma unroll
   for (int k = 0; k < K; k++) {
     *tab_at(1, k) = a[k];
     b[l * K + k] = a[k];
   }
-  __syncthreads();
+  wave_sync();
   for (uint32_t e = 2; e < T; e++) {
     mont.mul(a, a, b);
 #pragma unroll
     for (int k = 0; k < K; k++) *tab_at(e, k) = a[k];
   }
-  __syncthreads();
+  wave_sync();
 
   // exponent scan, top window first
   const uint32_t bits = args.exp.len * 32;
@@ -231,25 +242,25 @@ __device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, u
       } else {
 #pragma unroll
         for (int k = 0; k < K; k++) b[l * K + k] = a[k];
-        __syncthreads();
+        wave_sync();
         mont.mul(a, a, b);
-        __syncthreads();
+        wave_sync();
       }
     }
 #pragma unroll
     for (int k = 0; k < K; k++) b[l * K + k] = *tab_at(idx, k);
-    __syncthreads();
+    wave_sync();
     mont.mul(a, a, b);
-    __syncthreads();
+    wave_sync();
   }
 #undef tab_at
   // leave Montgomery form: a * 1 / R, which is <= m
 #pragma unroll
   for (int k = 0; k < K; k++) b[l * K + k] = (l == 0 && k == 0) ? 1u : 0u;
-  __syncthreads();
+  wave_sync();
   uint32_t r[K];
   mont.mul(r, a, b);
-  __syncthreads();
+  wave_sync();
   // canonical: r == m  ->  0
   bool ne = false;
 #pragma unroll
@@ -259,7 +270,7 @@ __device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, u
   const bool odd = (ld_limb(args.mod, mi, 0) & 1u) != 0;
 #pragma unroll
   for (int k = 0; k < K; k++) b[l * K + k] = (ne && odd) ? r[k] : 0u;
-  __syncthreads();
+  wave_sync();
   if (active) {
     const uint32_t oi = out_index(args.out, gi);
     for (uint32_t w = (uint32_t)l; w < args.out.len; w += L) {
@@ -317,6 +328,11 @@ const VariantEntry kVariants[] = {
     // above 8192 bits: half a wave / a whole wave per instance, radix 2^26
     XMP_VARIANT(32, 20, 26), // 16640 : 16384-bit
     XMP_VARIANT(64, 20, 26), // 33280 : 32768-bit
+    // low-latency layouts (fewer digits per lane, so one wave finishes sooner)
+    XMP_VARIANT(4, 10, 28),  // 1120 : 1024-bit on four lanes
+    XMP_VARIANT(8, 5, 28),   // 1120 : 1024-bit on eight lanes
+    XMP_VARIANT(8, 10, 28),  // 2240 : 2048-bit on eight lanes
+    XMP_VARIANT(16, 5, 28),  // 2240 : 2048-bit on sixteen lanes
 };
 #undef XMP_VARIANT
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
@@ -353,10 +369,11 @@ XMP_RUN(14, 16, 19, 27)
 XMP_RUN(15, 1, 5, 28)
 XMP_RUN(16, 1, 10, 28)
 #undef XMP_RUN
-// Variants 9 (2x55, which spills as a called function), 17 and 18 (whole-wave
-// layouts above 8192 bits) are not part of the persistent kernel.
-static_assert(kNumVariants == 19, "run_variant<> must list every batched variant in kVariants order");
-constexpr bool batched_variant(int i) { return i >= 0 && i < kNumVariants && i != 9 && i != 17 && i != 18; }
+// Variant 9 (2x55, which spills as a called function), 17 and 18 (whole-wave
+// layouts above 8192 bits) and the low-latency layouts 19..22 are not part of
+// the persistent kernel.
+static_assert(kNumVariants == 23, "run_variant<> must list every batched variant in kVariants order");
+constexpr bool batched_variant(int i) { return i >= 0 && i < 17 && i != 9; }
 
 __global__ void __launch_bounds__(64, 2) powm_batch_kernel(BatchArgs ba) {
   extern __shared__ __attribute__((aligned(16))) uint32_t dyn_lds[];
@@ -387,7 +404,7 @@ __global__ void __launch_bounds__(64, 2) powm_batch_kernel(BatchArgs ba) {
       case 16: run_variant<16>(args, tile, dyn_lds); break;
       default: break;
     }
-    __syncthreads();
+    wave_sync();
   }
 }
 

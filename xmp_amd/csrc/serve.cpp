@@ -75,10 +75,17 @@ struct _xmpServer_t {
   uint32_t bits = 0, n32 = 0, h32 = 0, nkeys = 0;
   // resident keys
   xmpIntegers_t N = nullptr, PQ = nullptr, DPQ = nullptr, CPQ = nullptr;
-  uint32_t *n_r2 = nullptr, *pq_r2 = nullptr;
-  uint32_t n_r2_stride = 0, pq_r2_stride = 0;
-  int var_n = -1, var_pq = -1;
-  uint32_t w_n = 0, w_pq = 0;
+  // Montgomery layouts per request kind, chosen per batch by its size: the
+  // tuner's choice at a few batch sizes up to the slot capacity, each with
+  // its own R^2 mod m (R depends on the layout)
+  struct Layout {
+    uint32_t count;   // batch size the choice was made for
+    int variant;
+    uint32_t window;
+    uint32_t *r2;     // shared between entries with the same variant
+    uint32_t r2_stride;
+  };
+  std::vector<Layout> lay_n, lay_pq;
   size_t bytes_per_request = 0;
   uint32_t max_inflight = 0;
 
@@ -128,6 +135,20 @@ Operand op(xmpIntegers_t x, const uint32_t *idx = nullptr, uint32_t idx_count = 
 }
 OutOperand outop(const DevArray &a) { return OutOperand{a.data, a.limbs, a.stride, nullptr}; }
 
+// the layout prepared for the batch size nearest (log scale) to `count`
+const _xmpServer_t::Layout &pick_layout(const std::vector<_xmpServer_t::Layout> &set, uint32_t count) {
+  size_t best = 0;
+  double bd = 1e30;
+  for (size_t i = 0; i < set.size(); i++) {
+    double d = std::fabs(std::log((double)set[i].count) - std::log((double)(count ? count : 1)));
+    if (d < bd) {
+      bd = d;
+      best = i;
+    }
+  }
+  return set[best];
+}
+
 // Launch one batch on its slot's stream: staging copies, the kernels for both
 // request kinds, result copy back and the completion event.
 hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
@@ -173,13 +194,14 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
     a.base = op(sl.a_base);
     a.exp = op(sl.a_exp);
     a.mod = op(s->N, d_keyA, na);
-    a.r2 = s->n_r2;
-    a.r2_stride = s->n_r2_stride;
+    const _xmpServer_t::Layout &ly = pick_layout(s->lay_n, na);
+    a.r2 = ly.r2;
+    a.r2_stride = ly.r2_stride;
     a.table = sl.table_a;
     a.start = 0;
     a.count = na;
-    a.window_bits = s->w_n;
-    if ((e = launch_powm_variant(s->var_n, a, st)) != hipSuccess) return e;
+    a.window_bits = ly.window;
+    if ((e = launch_powm_variant(ly.variant, a, st)) != hipSuccess) return e;
     if ((e = launch_export((uint8_t *)sl.d_out, sl.a_out.data, sl.a_out.stride, n32, na, n32, -1, -1, 0, 4, nullptr,
                            0, st)) != hipSuccess)
       return e;
@@ -197,13 +219,14 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
     a.base = op(sl.b_cm);
     a.exp = op(s->DPQ, d_keyPQ, 2 * nb);
     a.mod = op(s->PQ, d_keyPQ, 2 * nb);
-    a.r2 = s->pq_r2;
-    a.r2_stride = s->pq_r2_stride;
+    const _xmpServer_t::Layout &ly = pick_layout(s->lay_pq, 2 * nb);
+    a.r2 = ly.r2;
+    a.r2_stride = ly.r2_stride;
     a.table = sl.table_b;
     a.start = 0;
     a.count = 2 * nb;
-    a.window_bits = s->w_pq;
-    if ((e = launch_powm_variant(s->var_pq, a, st)) != hipSuccess) return e;
+    a.window_bits = ly.window;
+    if ((e = launch_powm_variant(ly.variant, a, st)) != hipSuccess) return e;
     // sp = mp*cp, sq = mq*cq; r = (sp + sq) mod n
     if ((e = launch_mul(outop(sl.b_prod), op(s->CPQ, d_keyPQ, 2 * nb), op(sl.b_pow), 2 * nb, st)) != hipSuccess)
       return e;
@@ -326,8 +349,15 @@ void free_server(xmpServer_t s) {
   }
   for (xmpIntegers_t x : {s->N, s->PQ, s->DPQ, s->CPQ})
     if (x) xmpIntegersDestroy(h, x);
-  if (s->n_r2) h->df(s->n_r2);
-  if (s->pq_r2) h->df(s->pq_r2);
+  for (auto *set : {&s->lay_n, &s->lay_pq}) {
+    std::vector<uint32_t *> freed;
+    for (auto &ly : *set)
+      if (ly.r2 && std::find(freed.begin(), freed.end(), ly.r2) == freed.end()) {
+        h->df(ly.r2);
+        freed.push_back(ly.r2);
+      }
+    set->clear();
+  }
 }
 
 }  // namespace
@@ -375,23 +405,37 @@ xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const
       (e = xmpIntegersImport(handle, s->CPQ, s->n32, -1, 4, 0, 0, cpq.data(), 2 * nkeys)))
     return fail(e);
   const uint32_t cap = config->max_batch;
-  PowmChoice cn, cpqc;
-  if ((e = choose_powm(handle, bits, bits, cap, xmpAlgorithmDefault, &cn)) ||
-      (e = choose_powm(handle, bits / 2, bits / 2, 2 * cap, xmpAlgorithmDefault, &cpqc)))
-    return fail(e);
-  s->var_n = cn.variant;
-  s->w_n = cn.window_bits;
-  s->var_pq = cpqc.variant;
-  s->w_pq = cpqc.window_bits;
-  const PowmVariant vn = powm_variant(cn.variant), vpq = powm_variant(cpqc.variant);
-  if ((e = montgomery_prepare(handle, s->N, vn, &s->n_r2, &s->n_r2_stride)) ||
-      (e = montgomery_prepare(handle, s->PQ, vpq, &s->pq_r2, &s->pq_r2_stride)))
-    return fail(e);
+  // candidate layouts: the tuner's choice at batch sizes 64, 256, ... up to the capacity
+  size_t tab_a = 0, tab_b = 0, req_a = 0, req_b = 0;
+  for (int kind = 0; kind < 2; kind++) {
+    auto &set = kind == 0 ? s->lay_n : s->lay_pq;
+    const uint32_t pbits = kind == 0 ? bits : bits / 2, mult = kind == 0 ? 1 : 2;
+    xmpIntegers_t mods = kind == 0 ? s->N : s->PQ;
+    std::vector<uint32_t> counts;
+    for (uint32_t c = 64; c < cap; c *= 4) counts.push_back(c * mult);
+    counts.push_back(cap * mult);
+    for (uint32_t c : counts) {
+      PowmChoice ch;
+      if ((e = choose_powm(handle, pbits, pbits, c, xmpAlgorithmDefault, &ch))) return fail(e);
+      _xmpServer_t::Layout ly{c, ch.variant, ch.window_bits, nullptr, 0};
+      for (auto &o : set)
+        if (o.variant == ch.variant) {
+          ly.r2 = o.r2;
+          ly.r2_stride = o.r2_stride;
+        }
+      if (!ly.r2 && (e = montgomery_prepare(handle, mods, powm_variant(ch.variant), &ly.r2, &ly.r2_stride)))
+        return fail(e);
+      set.push_back(ly);
+      const PowmVariant v = powm_variant(ch.variant);
+      size_t &tab = kind == 0 ? tab_a : tab_b;
+      size_t &req = kind == 0 ? req_a : req_b;
+      tab = std::max(tab, powm_table_words(v, round_up(cap * mult, kStrideAlign), ch.window_bits));
+      req = std::max(req, powm_table_words(v, mult, ch.window_bits) * 4);
+    }
+  }
   // device bytes one in-flight request holds (kind-1 worst case): staging, intermediates, table share
-  const size_t tab_a = powm_table_words(vn, 1, cn.window_bits) * 4;
-  const size_t tab_b = powm_table_words(vpq, 2, cpqc.window_bits) * 4;
   s->bytes_per_request = 4 * ((size_t)s->n32 * 8 + (size_t)s->h32 * 4 + 2 * ((size_t)s->n32 + s->h32) + s->n32 + 2) +
-                         std::max(tab_a, tab_b);
+                         std::max(req_a, req_b);
   size_t budget = config->hbm_budget ? config->hbm_budget : s->bytes_per_request * cap * config->streams;
   s->max_inflight = (uint32_t)std::max<size_t>(1, budget / s->bytes_per_request);
   s->slots.resize(config->streams);
@@ -406,9 +450,8 @@ xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const
     sl.d_in = (uint32_t *)handle->da(sizeof(uint32_t) * 3 * words);
     sl.d_idx = (uint32_t *)handle->da(sizeof(uint32_t) * kIdxWords * cap);
     sl.d_out = (uint32_t *)handle->da(sizeof(uint32_t) * 2 * words);
-    sl.table_a = (uint32_t *)handle->da(sizeof(uint32_t) * powm_table_words(vn, round_up(cap, kStrideAlign), cn.window_bits));
-    sl.table_b =
-        (uint32_t *)handle->da(sizeof(uint32_t) * powm_table_words(vpq, round_up(2 * cap, kStrideAlign), cpqc.window_bits));
+    sl.table_a = (uint32_t *)handle->da(sizeof(uint32_t) * tab_a);
+    sl.table_b = (uint32_t *)handle->da(sizeof(uint32_t) * tab_b);
     if (!sl.d_in || !sl.d_idx || !sl.d_out || !sl.table_a || !sl.table_b) return fail(xmpErrorInvalidCudaMalloc);
     if (alloc_array(handle, sl.a_base, s->n32, cap) || alloc_array(handle, sl.a_exp, s->n32, cap) ||
         alloc_array(handle, sl.a_out, s->n32, cap) || alloc_array(handle, sl.b_c, s->n32, cap) ||

@@ -30,7 +30,7 @@ PLAN = [
     (256, 256, [4096, 65536]),
     (512, 512, [4096, 65536, 262144]),
     (768, 768, [16384]),
-    (1024, 1024, [4096, 16384, 65536]),
+    (1024, 1024, [1024, 4096, 16384, 65536]),
     (1536, 1536, [16384]),
     (2048, 2048, [1024, 4096, 16384, 65536]),
     (3072, 3072, [4096, 16384]),
