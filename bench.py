@@ -21,6 +21,12 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The serving loop runs each batch slot on two streams (4 slots, 8 streams).
+# HIP maps streams onto this many hardware queues per process (default 4), so
+# raise it to at least 8 for the streams to run concurrently; it must be set
+# before the first HIP call.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 import xmp_amd  # noqa: E402
 from xmp_amd import hip, inputs  # noqa: E402

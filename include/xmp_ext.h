@@ -66,7 +66,9 @@ xmpError_t XMPAPI xmpBatchLaunchPlainAsync(xmpHandle_t handle, xmpBatch_t batch)
 typedef struct _xmpServer_t *xmpServer_t;
 
 typedef struct {
-  uint32_t streams;    /* batch slots in flight at once (each on its own stream) */
+  uint32_t streams;    /* batch slots in flight at once (each on two streams: the single powm
+                          requests and the CRT requests of a batch run concurrently; set
+                          GPU_MAX_HW_QUEUES >= 2 * streams for all of them to overlap) */
   uint32_t max_batch;  /* requests per batch slot */
   size_t hbm_budget;   /* device bytes for in-flight request state; 0 = streams * max_batch requests */
 } xmpServerConfig_t;

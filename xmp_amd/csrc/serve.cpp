@@ -90,8 +90,9 @@ struct _xmpServer_t {
   uint32_t max_inflight = 0;
 
   struct Slot {
-    hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    hipStream_t stream = nullptr;     // staging and the kind-0 (single powm) work
+    hipStream_t stream_b = nullptr;   // the kind-1 (CRT) work, concurrent with kind 0
+    hipEvent_t done = nullptr, fork = nullptr, join = nullptr;
     uint32_t cap = 0;
     // pinned staging
     uint32_t *h_in = nullptr;     // [A base | A exp | B c] compact, cap*n32 each
@@ -149,8 +150,10 @@ const _xmpServer_t::Layout &pick_layout(const std::vector<_xmpServer_t::Layout> 
   return set[best];
 }
 
-// Launch one batch on its slot's stream: staging copies, the kernels for both
-// request kinds, result copy back and the completion event.
+// Launch one batch on its slot's streams: the index staging, then the two
+// request kinds concurrently (kind 0 on the slot's stream, kind 1 on its
+// second stream, each with its own input copy, kernels and result copy), then
+// the completion event once both have finished.
 hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
   const uint32_t n32 = s->n32, h32 = s->h32;
   const uint32_t na = sl.na, nb = sl.nb, cap = sl.cap;
@@ -174,6 +177,8 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
   hipError_t e;
   if ((e = hipMemcpyAsync(sl.d_idx, sl.h_idx, sizeof(uint32_t) * kIdxWords * cap, hipMemcpyHostToDevice, st)) != hipSuccess)
     return e;
+  hipStream_t sb = sl.stream_b;
+  if ((e = hipEventRecord(sl.fork, st)) || (e = hipStreamWaitEvent(sb, sl.fork, 0))) return e;
   const size_t words = (size_t)cap * n32;
   // only the rows this batch uses: bases and exponents of the na powm requests, c of the nb CRT requests
   if (na && ((e = hipMemcpyAsync(sl.d_in, sl.h_in, sizeof(uint32_t) * na * n32, hipMemcpyHostToDevice, st)) ||
@@ -181,7 +186,7 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
                                  hipMemcpyHostToDevice, st))))
     return e;
   if (nb && (e = hipMemcpyAsync(sl.d_in + 2 * words, sl.h_in + 2 * words, sizeof(uint32_t) * nb * n32,
-                                hipMemcpyHostToDevice, st)))
+                                hipMemcpyHostToDevice, sb)))
     return e;
   const uint8_t *in_base = (const uint8_t *)sl.d_in, *in_exp = (const uint8_t *)(sl.d_in + words),
                 *in_c = (const uint8_t *)(sl.d_in + 2 * words);
@@ -207,11 +212,11 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
       return e;
   }
   if (nb) {
-    if ((e = launch_import(sl.b_c.data, sl.b_c.stride, n32, in_c, nb, n32, -1, -1, 0, 4, nullptr, 0, st))) return e;
+    if ((e = launch_import(sl.b_c.data, sl.b_c.stride, n32, in_c, nb, n32, -1, -1, 0, 4, nullptr, 0, sb))) return e;
     // c mod p_k and c mod q_k in one launch over 2*nb instances
     OutOperand none{nullptr, 0, 0, nullptr};
     if ((e = launch_divmod_ws(none, outop(sl.b_cm), op(sl.b_c, d_cidx, 2 * nb), op(s->PQ, d_keyPQ, 2 * nb), 2 * nb,
-                              nullptr, st)) != hipSuccess)
+                              nullptr, sb)) != hipSuccess)
       return e;
     // mp = cm^dp mod p, mq = cm^dq mod q in one launch
     PowmArgs a;
@@ -226,24 +231,25 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
     a.start = 0;
     a.count = 2 * nb;
     a.window_bits = ly.window;
-    if ((e = launch_powm_variant(ly.variant, a, st)) != hipSuccess) return e;
+    if ((e = launch_powm_variant(ly.variant, a, sb)) != hipSuccess) return e;
     // sp = mp*cp, sq = mq*cq; r = (sp + sq) mod n
-    if ((e = launch_mul(outop(sl.b_prod), op(s->CPQ, d_keyPQ, 2 * nb), op(sl.b_pow), 2 * nb, st)) != hipSuccess)
+    if ((e = launch_mul(outop(sl.b_prod), op(s->CPQ, d_keyPQ, 2 * nb), op(sl.b_pow), 2 * nb, sb)) != hipSuccess)
       return e;
-    if ((e = launch_add(outop(sl.b_sum), op(sl.b_prod, d_lo, nb), op(sl.b_prod, d_hi, nb), nb, st)) != hipSuccess)
+    if ((e = launch_add(outop(sl.b_sum), op(sl.b_prod, d_lo, nb), op(sl.b_prod, d_hi, nb), nb, sb)) != hipSuccess)
       return e;
-    if ((e = launch_divmod_ws(none, outop(sl.b_out), op(sl.b_sum), op(s->N, d_keyB, nb), nb, nullptr, st)) !=
+    if ((e = launch_divmod_ws(none, outop(sl.b_out), op(sl.b_sum), op(s->N, d_keyB, nb), nb, nullptr, sb)) !=
         hipSuccess)
       return e;
     if ((e = launch_export((uint8_t *)(sl.d_out + words), sl.b_out.data, sl.b_out.stride, n32, nb, n32, -1, -1, 0, 4,
-                           nullptr, 0, st)) != hipSuccess)
+                           nullptr, 0, sb)) != hipSuccess)
       return e;
   }
   if (na && (e = hipMemcpyAsync(sl.h_out, sl.d_out, sizeof(uint32_t) * na * n32, hipMemcpyDeviceToHost, st)))
     return e;
   if (nb && (e = hipMemcpyAsync(sl.h_out + words, sl.d_out + words, sizeof(uint32_t) * nb * n32,
-                                hipMemcpyDeviceToHost, st)))
+                                hipMemcpyDeviceToHost, sb)))
     return e;
+  if ((e = hipEventRecord(sl.join, sb)) || (e = hipStreamWaitEvent(st, sl.join, 0))) return e;
   (void)h32;
   return hipEventRecord(sl.done, st);
 }
@@ -333,6 +339,7 @@ void free_server(xmpServer_t s) {
   xmpHandle_t h = s->h;
   for (auto &sl : s->slots) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.stream_b) (void)hipStreamSynchronize(sl.stream_b);
     for (DevArray *a : {&sl.a_base, &sl.a_exp, &sl.a_out, &sl.b_c, &sl.b_cm, &sl.b_pow, &sl.b_prod, &sl.b_sum,
                         &sl.b_out})
       if (a->data) h->df(a->data);
@@ -344,8 +351,10 @@ void free_server(xmpServer_t s) {
     if (sl.h_in) (void)hipHostFree(sl.h_in);
     if (sl.h_idx) (void)hipHostFree(sl.h_idx);
     if (sl.h_out) (void)hipHostFree(sl.h_out);
-    if (sl.done) (void)hipEventDestroy(sl.done);
+    for (hipEvent_t ev : {sl.done, sl.fork, sl.join})
+      if (ev) (void)hipEventDestroy(ev);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    if (sl.stream_b) (void)hipStreamDestroy(sl.stream_b);
   }
   for (xmpIntegers_t x : {s->N, s->PQ, s->DPQ, s->CPQ})
     if (x) xmpIntegersDestroy(h, x);
@@ -442,7 +451,10 @@ xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const
   for (auto &sl : s->slots) {
     sl.cap = cap;
     const size_t words = (size_t)cap * s->n32;
-    if (hipStreamCreate(&sl.stream) != hipSuccess || hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
+    if (hipStreamCreate(&sl.stream) != hipSuccess || hipStreamCreate(&sl.stream_b) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.join, hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void **)&sl.h_in, sizeof(uint32_t) * 3 * words, 0) != hipSuccess ||
         hipHostMalloc((void **)&sl.h_idx, sizeof(uint32_t) * kIdxWords * cap, 0) != hipSuccess ||
         hipHostMalloc((void **)&sl.h_out, sizeof(uint32_t) * 2 * words, 0) != hipSuccess)
