@@ -77,7 +77,7 @@ struct Mont {
     uint32_t q = ((uint32_t)f * np0) & MASK;
     q = g.bcast0(q);
     f += (uint64_t)q * m[0];
-    const uint32_t up = g.from_next((uint32_t)f & MASK);
+    const uint32_t up = g.from_next_z((uint32_t)f & MASK);
 #pragma unroll
     for (int j = 1; j < K; j++) t[j - 1] = t[j] + (uint64_t)a[j] * bi + (uint64_t)q * m[j];
     t[K - 1] = up;
@@ -149,15 +149,17 @@ struct Mont {
   }
 
   // Symmetric squaring on a lane group: r = a^2 / R (mod m), with a's digits
-  // also staged in LDS at b (the step multipliers).  Step i = li*K + KI adds
-  // a_x a_y for each pair once, at the step of x or of y, so that every lane
-  // does the same number of multiply-adds per step:
+  // also staged in LDS at b (the step multipliers).  Step i = li*K + KI
+  // multiplies digit i with the lanes' own digits; each pair x != y must
+  // contribute 2 a_x a_y in total, and is split over the steps of x and y so
+  // that every lane does the same number of multiply-adds per step:
   //  - pairs in different register slots (x mod K != y mod K): at the step of
   //    x when slot(x) beats slot(y) in a regular tournament on the K slots
   //    (slot s beats s+1 .. s+(K-1)/2 cyclically; for even K, s also beats
   //    s+K/2 when s < K/2), as 2 a_x a_y;
-  //  - pairs in one slot on two lanes: at the step of the lower lane's digit,
-  //    as 2 a_x a_y, and the square a_x^2 at step x.
+  //  - pairs in one slot on two lanes: at both steps, as a_x a_y each time,
+  //    and the square a_x^2 at step x, so that slot multiplies by a_i on
+  //    every lane.
   // A pair (x, y) lands at position x + y >= max(x, y), never below the
   // position its step retires, so every position holds the same total as in
   // the plain product when it is retired: quotients, carries and the result
@@ -167,7 +169,18 @@ struct Mont {
     uint64_t t[K];
 #pragma unroll
     for (int k = 0; k < K; k++) t[k] = 0;
-    for (int li = 0; li < L; li++) gsqr_steps(t, a, b + li * K, li, std::make_integer_sequence<int, K>());
+    // one lane's K steps are unrolled; the loop over lanes is kept (unrolling
+    // it too only raises register pressure)
+#pragma unroll 1
+    for (int li = 0; li < L; li++) {
+      uint32_t ac[K];
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        ac[k] = a[k];
+        asm("" : "+v"(ac[k]));
+      }
+      gsqr_steps(t, ac, b + li * K, std::make_integer_sequence<int, K>());
+    }
     normalize(r, t);
   }
 
@@ -177,41 +190,38 @@ struct Mont {
   }
 
   template <int... I>
-  __device__ __forceinline__ void gsqr_steps(uint64_t (&t)[K], const uint32_t (&a)[K], const uint32_t *bl, int li,
+  __device__ __forceinline__ void gsqr_steps(uint64_t (&t)[K], const uint32_t (&a)[K], const uint32_t *bl,
                                              std::integer_sequence<int, I...>) const {
-    (gsqr_step<I>(t, a, bl, li), ...);
+    (gsqr_step<I>(t, a, bl), ...);
   }
 
   template <int KI>
-  __device__ __forceinline__ void gsqr_step(uint64_t (&t)[K], const uint32_t (&a)[K], const uint32_t *bl,
-                                            int li) const {
+  __device__ __forceinline__ void gsqr_step(uint64_t (&t)[K], const uint32_t (&a)[K], const uint32_t *bl) const {
     const uint32_t bi = bl[KI];
     const uint32_t b2 = bi << 1;
-    // slot KI: the square on the step's own lane, doubled pairs on the lanes above it
-    const uint32_t bd = g.l == li ? bi : (g.l > li ? b2 : 0u);
     uint64_t f = t[0];
-    if constexpr (KI == 0) f += (uint64_t)a[0] * bd;
+    if constexpr (KI == 0) f += (uint64_t)a[0] * bi;
     else if constexpr (beats(KI, 0)) f += (uint64_t)a[0] * b2;
     uint32_t q = ((uint32_t)f * np0) & MASK;
     q = g.bcast0(q);
     f += (uint64_t)q * m[0];
-    const uint32_t up = g.from_next((uint32_t)f & MASK);
-    gsqr_cols<KI>(t, a, q, b2, bd, std::make_integer_sequence<int, K - 1>());
+    const uint32_t up = g.from_next_z((uint32_t)f & MASK);
+    gsqr_cols<KI>(t, a, q, bi, b2, std::make_integer_sequence<int, K - 1>());
     t[K - 1] = up;
     t[0] += f >> D;
   }
 
   template <int KI, int... J>
-  __device__ __forceinline__ void gsqr_cols(uint64_t (&t)[K], const uint32_t (&a)[K], uint32_t q, uint32_t b2,
-                                            uint32_t bd, std::integer_sequence<int, J...>) const {
-    (gsqr_col<KI, J + 1>(t, a, q, b2, bd), ...);
+  __device__ __forceinline__ void gsqr_cols(uint64_t (&t)[K], const uint32_t (&a)[K], uint32_t q, uint32_t bi,
+                                            uint32_t b2, std::integer_sequence<int, J...>) const {
+    (gsqr_col<KI, J + 1>(t, a, q, bi, b2), ...);
   }
 
   template <int KI, int J>
-  __device__ __forceinline__ void gsqr_col(uint64_t (&t)[K], const uint32_t (&a)[K], uint32_t q, uint32_t b2,
-                                           uint32_t bd) const {
+  __device__ __forceinline__ void gsqr_col(uint64_t (&t)[K], const uint32_t (&a)[K], uint32_t q, uint32_t bi,
+                                           uint32_t b2) const {
     uint64_t x = t[J] + (uint64_t)q * m[J];
-    if constexpr (J == KI) x += (uint64_t)a[J] * bd;
+    if constexpr (J == KI) x += (uint64_t)a[J] * bi;
     else if constexpr (beats(KI, J)) x += (uint64_t)a[J] * b2;
     t[J - 1] = x;
   }

@@ -60,6 +60,19 @@ struct Group {
       return l == L - 1 ? 0u : v;
     }
   }
+  // from_next for a value that is zero on the group's first lane (the
+  // Montgomery step's retired digit): the move may rotate inside the group,
+  // the last lane reading the first lane's zero, which needs no select for
+  // L = 2 (quad_perm [1,0,3,2] = 0xB1) and L = 4 (quad_perm [1,2,3,0] = 0x39)
+  __device__ __forceinline__ uint32_t from_next_z(uint32_t x) const {
+    if constexpr (L == 2) {
+      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);
+    } else if constexpr (L == 4) {
+      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x39, 0xF, 0xF, false);
+    } else {
+      return from_next(x);
+    }
+  }
   __device__ __forceinline__ uint64_t from_prev(uint64_t x) const {
     if constexpr (L == 1) {
       return 0;
