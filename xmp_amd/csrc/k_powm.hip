@@ -63,8 +63,15 @@ struct Mont {
   Group<L> g;
   uint32_t m[K];
   uint32_t np0;
+  // the digit mask held in a register: an AND with a register operand can
+  // take a DPP move of its other operand as a modifier, so the quotient's
+  // broadcast and the digit hand-over each fold into their mask instruction
+  uint32_t mask;
 
-  __device__ __forceinline__ explicit Mont(int lane) : g(lane) {}
+  __device__ __forceinline__ explicit Mont(int lane) : g(lane) {
+    uint32_t v = MASK;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(mask) : "s"(v));
+  }
 
   // One CIOS step: t = (t + a*b_i + q*m) / 2^D with q making the lowest digit
   // vanish.  Lane l's first accumulator f (position l*K) is split as
@@ -74,10 +81,9 @@ struct Mont {
   // crosses lanes.
   __device__ __forceinline__ void step(uint64_t (&t)[K], const uint32_t (&a)[K], uint32_t bi) const {
     uint64_t f = t[0] + (uint64_t)a[0] * bi;
-    uint32_t q = ((uint32_t)f * np0) & MASK;
-    q = g.bcast0(q);
+    const uint32_t q = g.bcast0((uint32_t)f * np0) & mask;
     f += (uint64_t)q * m[0];
-    const uint32_t up = g.from_next_z((uint32_t)f & MASK);
+    const uint32_t up = g.from_next_z((uint32_t)f) & mask;
 #pragma unroll
     for (int j = 1; j < K; j++) t[j - 1] = t[j] + (uint64_t)a[j] * bi + (uint64_t)q * m[j];
     t[K - 1] = up;
@@ -173,6 +179,9 @@ struct Mont {
     // it too only raises register pressure)
 #pragma unroll 1
     for (int li = 0; li < L; li++) {
+      // fresh copies behind an empty asm each iteration, so the compiler
+      // cannot hoist the digits' 64-bit extensions out of the loop (that
+      // doubles the registers they take and spills)
       uint32_t ac[K];
 #pragma unroll
       for (int k = 0; k < K; k++) {
@@ -202,10 +211,9 @@ struct Mont {
     uint64_t f = t[0];
     if constexpr (KI == 0) f += (uint64_t)a[0] * bi;
     else if constexpr (beats(KI, 0)) f += (uint64_t)a[0] * b2;
-    uint32_t q = ((uint32_t)f * np0) & MASK;
-    q = g.bcast0(q);
+    const uint32_t q = g.bcast0((uint32_t)f * np0) & mask;
     f += (uint64_t)q * m[0];
-    const uint32_t up = g.from_next_z((uint32_t)f & MASK);
+    const uint32_t up = g.from_next_z((uint32_t)f) & mask;
     gsqr_cols<KI>(t, a, q, bi, b2, std::make_integer_sequence<int, K - 1>());
     t[K - 1] = up;
     t[0] += f >> D;

@@ -23,6 +23,9 @@ namespace xmp {
 // the next row), wave_shl:1 = 0x130 and wave_shr:1 = 0x138 (whole-wave shifts,
 // for groups that span rows).  A 64-lane group reads its first lane with
 // readfirstlane, which leaves the quotient digit in a scalar register.
+// Moves whose source lane is always valid (quad_perm, row_newbcast) are
+// still marked bound_ctrl: the compiler folds such a move into the VALU
+// instruction that uses it (as a DPP operand) only when it is.
 template <int L>
 struct Group {
   static_assert(L == 1 || L == 2 || L == 4 || L == 8 || L == 16 || L == 32 || L == 64, "group size");
@@ -34,15 +37,15 @@ struct Group {
     if constexpr (L == 1) {
       return x;
     } else if constexpr (L == 2) {
-      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xA0, 0xF, 0xF, false);
+      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xA0, 0xF, 0xF, true);
     } else if constexpr (L == 4) {
-      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x00, 0xF, 0xF, false);
+      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x00, 0xF, 0xF, true);
     } else if constexpr (L == 8) {
       uint32_t y = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x00, 0xF, 0xF, false);
       uint32_t z = (uint32_t)__builtin_amdgcn_mov_dpp((int)y, 0x114, 0xF, 0xF, true);
       return (l & 4) ? z : y;
     } else if constexpr (L == 16) {
-      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x150, 0xF, 0xF, false);
+      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x150, 0xF, 0xF, true);
     } else if constexpr (L == 32) {
       // every row takes its lane 0; rows 1 and 3 then take lane 15 of rows 0 and 2
       uint32_t y = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x150, 0xF, 0xF, false);
@@ -66,9 +69,9 @@ struct Group {
   // L = 2 (quad_perm [1,0,3,2] = 0xB1) and L = 4 (quad_perm [1,2,3,0] = 0x39)
   __device__ __forceinline__ uint32_t from_next_z(uint32_t x) const {
     if constexpr (L == 2) {
-      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);
+      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, true);
     } else if constexpr (L == 4) {
-      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x39, 0xF, 0xF, false);
+      return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x39, 0xF, 0xF, true);
     } else {
       return from_next(x);
     }
