@@ -56,6 +56,14 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// low 32 bits of x * y, issued as v_mad_u64_u32 (the multiply the kernels
+// are built on) instead of v_mul_lo_u32
+__device__ __forceinline__ uint32_t mul_lo_mad(uint32_t x, uint32_t y) {
+  uint64_t r, c;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, 0" : "=v"(r), "=s"(c) : "v"(x), "v"(y));
+  return (uint32_t)r;
+}
+
 template <int L, int K, int D>
 struct Mont {
   static constexpr int ND = L * K;
@@ -81,7 +89,7 @@ struct Mont {
   // crosses lanes.
   __device__ __forceinline__ void step(uint64_t (&t)[K], const uint32_t (&a)[K], uint32_t bi) const {
     uint64_t f = t[0] + (uint64_t)a[0] * bi;
-    const uint32_t q = g.bcast0((uint32_t)f * np0) & mask;
+    const uint32_t q = g.bcast0(mul_lo_mad((uint32_t)f, np0)) & mask;
     f += (uint64_t)q * m[0];
     const uint32_t up = g.from_next_z((uint32_t)f) & mask;
 #pragma unroll
@@ -141,7 +149,7 @@ struct Mont {
     const uint32_t ai = a[I], ai2 = a[I] << 1;
     uint64_t f = t[0];
     if constexpr (I == 0) f += (uint64_t)ai * ai;
-    const uint32_t q = ((uint32_t)f * np0) & MASK;
+    const uint32_t q = mul_lo_mad((uint32_t)f, np0) & MASK;
     f += (uint64_t)q * m[0];
 #pragma unroll
     for (int j = 1; j < K; j++) {
@@ -211,7 +219,7 @@ struct Mont {
     uint64_t f = t[0];
     if constexpr (KI == 0) f += (uint64_t)a[0] * bi;
     else if constexpr (beats(KI, 0)) f += (uint64_t)a[0] * b2;
-    const uint32_t q = g.bcast0((uint32_t)f * np0) & mask;
+    const uint32_t q = g.bcast0(mul_lo_mad((uint32_t)f, np0)) & mask;
     f += (uint64_t)q * m[0];
     const uint32_t up = g.from_next_z((uint32_t)f) & mask;
     gsqr_cols<KI>(t, a, q, bi, b2, std::make_integer_sequence<int, K - 1>());
