@@ -23,11 +23,14 @@
 //  * an instance spans L lanes of the wave (L = 1 ... 64), each holding K
 //    digits of the multiplicand, the modulus and the accumulator in
 //    registers.  Per CIOS step the quotient digit is broadcast from the
-//    group's first lane and one 28-bit digit moves one lane down, both with
-//    DPP moves;
+//    group's first lane and one 28-bit digit moves one lane down, both as
+//    DPP operands of the instructions that mask them;
 //  * the multiplier (the current value when squaring, a table entry when
-//    multiplying) is staged in LDS and read four digits at a time with an
-//    LDS broadcast;
+//    multiplying) is staged in LDS and read with LDS broadcasts;
+//  * squarings compute most products of two different digits once and
+//    double them: within one lane as a triangle, across a lane group by a
+//    tournament on register slots that gives every lane the same share
+//    (K + K/2 + 1 multiply-adds per step instead of 2K);
 //  * the 2^w window table lives in HBM, one contiguous block per instance.
 //
 // The exponent is scanned over all of its precision's bits with a fixed
