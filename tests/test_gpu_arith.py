@@ -186,7 +186,8 @@ def test_mul_closed_form_and_square(handle):
 
 
 @pytest.mark.parametrize("bits_a,bits_b", [(64, 32), (128, 64), (512, 256), (1024, 1024), (4096, 2048),
-                                           (2048, 96), (8192, 4096), (3072, 3072)])
+                                           (2048, 96), (8192, 4096), (3072, 3072), (8192, 3104),
+                                           (16384, 8192)])
 def test_div_mod(handle, bits_a, bits_b):
     n = 700
     a = inputs.uniform(61, n, bits_a // 32)
@@ -206,25 +207,27 @@ def test_div_mod(handle, bits_a, bits_b):
     assert_rows_equal(R2.get(), wr, "divmod r")
 
 
-def test_div_short_and_unnormalised_divisors(handle):
-    """Divisors with leading zero limbs / bits, a < b, and b = 1."""
-    n = 800
-    a = inputs.uniform(71, n, 128)
-    b = inputs.uniform(72, n, 64)
+@pytest.mark.parametrize("nb", [64, 128, 192])
+def test_div_short_and_unnormalised_divisors(handle, nb):
+    """Divisors with leading zero limbs / bits, a < b, and b = 1, for the register
+    kernels (64 and 128 limbs) and the workspace kernel (192)."""
+    n = 800 if nb <= 64 else 256
+    a = inputs.uniform(71, n, 2 * nb)
+    b = inputs.uniform(72, n, nb)
     rng = np.random.default_rng(9)
     for i in range(n):
-        top = int(rng.integers(1, 65))
+        top = int(rng.integers(1, nb + 1))
         b[i, top:] = 0
         b[i, top - 1] >>= int(rng.integers(0, 32))
         if not b[i].any():
             b[i, 0] = 1
-    a[: n // 8, 64:] = 0   # some a < b
+    a[: n // 8, nb:] = 0   # some a < b
     b[n - 5:] = 0
     b[n - 5:, 0] = 1
     A, B = handle.from_array(a), handle.from_array(b)
-    Q, R = handle.integers(4096, n), handle.integers(2048, n)
+    Q, R = handle.integers(64 * nb, n), handle.integers(32 * nb, n)
     handle.divmod(Q, R, A, B)
-    wq, wr = gmp(O.GO_DIV, n, 128, a, b, out2_limbs=64)
+    wq, wr = gmp(O.GO_DIV, n, 2 * nb, a, b, out2_limbs=nb)
     assert_rows_equal(Q.get(), wq, "div q")
     assert_rows_equal(R.get(), wr, "div r")
 

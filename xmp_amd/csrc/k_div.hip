@@ -429,6 +429,8 @@ hipError_t launch_divmod_ws(OutOperand q, OutOperand r, Operand a, Operand b, ui
   if (nb <= 32) return launch_divmod_reg<32>(q, r, a, b, count, stream);
   if (nb <= 48) return launch_divmod_reg<48>(q, r, a, b, count, stream);
   if (nb <= 64) return launch_divmod_reg<64>(q, r, a, b, count, stream);
+  if (nb <= 96) return launch_divmod_reg<96>(q, r, a, b, count, stream);
+  if (nb <= 128) return launch_divmod_reg<128>(q, r, a, b, count, stream);
   if (!scratch) return hipErrorInvalidValue;
   hipLaunchKernelGGL(divmod_mem_kernel, dim3(div_round_up(count, 256)), dim3(256), 0, stream, q, r, a, b, count,
                      scratch, round_up(count, 256));
@@ -441,6 +443,8 @@ hipError_t launch_pow2_mod(uint32_t *out, uint32_t out_len, uint32_t out_stride,
   if (m_len <= 16) return launch_pow2_reg<16>(out, out_len, out_stride, m, m_len, m_stride, count, e, stream);
   if (m_len <= 32) return launch_pow2_reg<32>(out, out_len, out_stride, m, m_len, m_stride, count, e, stream);
   if (m_len <= 64) return launch_pow2_reg<64>(out, out_len, out_stride, m, m_len, m_stride, count, e, stream);
+  if (m_len <= 96) return launch_pow2_reg<96>(out, out_len, out_stride, m, m_len, m_stride, count, e, stream);
+  if (m_len <= 128) return launch_pow2_reg<128>(out, out_len, out_stride, m, m_len, m_stride, count, e, stream);
   if (!scratch) return hipErrorInvalidValue;
   hipLaunchKernelGGL(pow2_mod_mem_kernel, dim3(div_round_up(count, 256)), dim3(256), 0, stream, out, out_len,
                      out_stride, m, m_len, m_stride, count, e, scratch, round_up(count, 256));

@@ -10,7 +10,7 @@
 
 namespace xmp {
 
-constexpr uint32_t kMaxRegDivisorLimbs = 64;
+constexpr uint32_t kMaxRegDivisorLimbs = 128;
 
 // division workspace (words) for divisors beyond the register kernels
 size_t divmod_scratch_words(uint32_t b_len, uint32_t count);
