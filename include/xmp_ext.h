@@ -36,6 +36,11 @@ xmpError_t XMPAPI xmpMontgomeryDestroy(xmpHandle_t handle, xmpMontgomery_t ctx);
 xmpError_t XMPAPI xmpMontgomeryGetLayout(xmpHandle_t handle, xmpMontgomery_t ctx, uint32_t *lanes,
                                          uint32_t *digits, uint32_t *radix, uint32_t *window_bits);
 
+/* copy the context's resident R^2 mod m (R = 2^(lanes*digits*radix) of its
+ * layout) into `out` (same precision as the moduli, count >= theirs); waits
+ * for the copy.  For inspection and tests. */
+xmpError_t XMPAPI xmpMontgomeryGetR2(xmpHandle_t handle, xmpMontgomery_t ctx, xmpIntegers_t out);
+
 /* out = base^exp mod m using a context: one kernel launch on the handle's
  * stream.  Index semantics as xmpIntegersPowm (policy slots 0..3; slot 3
  * selects the modulus inside the context's array). */

@@ -146,6 +146,29 @@ def test_powm_fermat_and_crt_sample_key(handle):
         assert O.to_int(got[i]) == want
 
 
+@pytest.mark.parametrize("bits,n", [(2048, 64), (4096, 64), (6144, 32), (8192, 32), (16384, 8), (32768, 4)])
+def test_montgomery_r2_canonical(handle, bits, n):
+    """The resident R^2 mod m of a context is canonical (< m) and exact: one
+    division up to 4096-bit moduli, Montgomery squarings of 2^t R mod m above.
+    Checked against Python's pow(2, 2 rbits, m) on every modulus."""
+    from xmp_amd.ext import Montgomery
+    m = inputs.odd_moduli(4100 + bits, n, bits // 32)
+    m[0] = 0
+    m[0, 0] = 1 + 2          # a tiny modulus in a wide container
+    m[1, -1] = 0x80000000    # top limb with only its top bit
+    m[2, :] = 0xFFFFFFFF     # 2^bits - 1
+    M = handle.from_array(m)
+    ctx = Montgomery(handle, M, bits, n)
+    try:
+        lanes, digits, radix, _ = ctx.layout()
+        rbits = lanes * digits * radix
+        got = ctx.r2()
+    finally:
+        ctx.close()
+    want = np.stack([O.from_int(pow(2, 2 * rbits, O.to_int(row)), bits // 32) for row in m])
+    assert_rows_equal(got, want, "R^2 mod m, %d-bit moduli, R = 2^%d" % (bits, rbits))
+
+
 def test_powm_indices_and_resident_moduli(handle):
     """Policy index gather (slots 1..3) and scatter (slot 0), as the reference
     powm kernels read them (powm_implementations.h:417-418, copy-out :480-516)."""

@@ -21,7 +21,7 @@
 #include "tuner.hpp"
 
 namespace xmp {
-xmpError_t montgomery_prepare(xmpHandle_t h, xmpIntegers_t mod, const PowmVariant &v, uint32_t **r2_out,
+xmpError_t montgomery_prepare(xmpHandle_t h, xmpIntegers_t mod, int variant, uint32_t **r2_out,
                               uint32_t *stride_out);
 }
 
@@ -121,7 +121,7 @@ xmpError_t XMPAPI xmpBatchCreate(xmpHandle_t handle, xmpBatch_t *batch, uint32_t
       e = xmpErrorInvalidCudaMalloc;
       break;
     }
-    e = montgomery_prepare(handle, mod[c], v, &b->r2[c], &b->r2_stride[c]);
+    e = montgomery_prepare(handle, mod[c], ch.variant, &b->r2[c], &b->r2_stride[c]);
     if (e != xmpErrorSuccess) break;
     PowmArgs &a = args[c];
     a.out = OutOperand{out[c]->limbs, out[c]->nlimbs, out[c]->stride, nullptr};
@@ -193,15 +193,14 @@ xmpError_t XMPAPI xmpBatchPrepare(xmpHandle_t handle, xmpBatch_t batch) {
   XMP_SET_DEVICE(handle);
   for (uint32_t c = 0; c < batch->classes; c++) {
     xmpIntegers_t m = batch->mod[c];
-    PowmVariant v = powm_variant(batch->variant[c]);
     uint32_t *ws = nullptr;
     if (m->nlimbs > kMaxRegDivisorLimbs) {
       xmpError_t e = set_necessary_scratch(handle, (size_t)(2 * m->nlimbs + 1) * round_up(m->count, 256) * 4);
       if (e != xmpErrorSuccess) return e;
       ws = (uint32_t *)handle->scratch;
     }
-    if (launch_pow2_mod(batch->r2[c], m->nlimbs, batch->r2_stride[c], m->limbs, m->nlimbs, m->stride, m->count,
-                        2 * variant_rbits(v), ws, handle->stream) != hipSuccess)
+    if (launch_montgomery_r2(batch->variant[c], batch->r2[c], batch->r2_stride[c], m->limbs, m->nlimbs, m->stride,
+                             m->count, ws, handle->stream) != hipSuccess)
       return xmpErrorCuda;
   }
   XMP_CHECK_HIP();

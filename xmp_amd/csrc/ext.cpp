@@ -32,7 +32,7 @@ struct _xmpMontgomery_t {
 namespace xmp {
 
 // R^2 mod m for every instance of `mod` into a new device buffer
-xmpError_t montgomery_prepare(xmpHandle_t h, xmpIntegers_t mod, const PowmVariant &v, uint32_t **r2_out,
+xmpError_t montgomery_prepare(xmpHandle_t h, xmpIntegers_t mod, int variant, uint32_t **r2_out,
                               uint32_t *stride_out) {
   const uint32_t stride = round_up(mod->count, kStrideAlign);
   uint32_t *r2 = (uint32_t *)h->da(sizeof(uint32_t) * (size_t)mod->nlimbs * stride);
@@ -47,8 +47,8 @@ xmpError_t montgomery_prepare(xmpHandle_t h, xmpIntegers_t mod, const PowmVarian
     }
     ws = (uint32_t *)h->scratch;
   }
-  if (launch_pow2_mod(r2, mod->nlimbs, stride, mod->limbs, mod->nlimbs, mod->stride, mod->count,
-                      2 * variant_rbits(v), ws, h->stream) != hipSuccess) {
+  if (launch_montgomery_r2(variant, r2, stride, mod->limbs, mod->nlimbs, mod->stride, mod->count, ws, h->stream) !=
+      hipSuccess) {
     h->df(r2);
     return xmpErrorCuda;
   }
@@ -79,7 +79,7 @@ xmpError_t XMPAPI xmpMontgomeryCreate(xmpHandle_t handle, xmpMontgomery_t *ctx, 
   m->variant = c.variant;
   m->window_bits = c.window_bits;
   m->exp_precision = exp_precision;
-  e = montgomery_prepare(handle, mod, powm_variant(c.variant), &m->r2, &m->r2_stride);
+  e = montgomery_prepare(handle, mod, c.variant, &m->r2, &m->r2_stride);
   if (e != xmpErrorSuccess) {
     m->~_xmpMontgomery_t();
     handle->hf(m);
@@ -111,6 +111,22 @@ xmpError_t XMPAPI xmpMontgomeryGetLayout(xmpHandle_t handle, xmpMontgomery_t ctx
   if (digits) *digits = (uint32_t)v.digits;
   if (radix) *radix = (uint32_t)v.radix;
   if (window_bits) *window_bits = ctx->window_bits;
+  return xmpErrorSuccess;
+}
+
+xmpError_t XMPAPI xmpMontgomeryGetR2(xmpHandle_t handle, xmpMontgomery_t ctx, xmpIntegers_t out) {
+  XMP_CHECK_NE(ctx, nullptr);
+  XMP_CHECK_NE(out, nullptr);
+  if (out->device != handle->device) return xmpErrorInvalidDevice;
+  const xmpIntegers_t m = ctx->mod;
+  if (out->precision != m->precision) return xmpErrorInvalidPrecision;
+  if (out->count < m->count) return xmpErrorInvalidCount;
+  XMP_SET_DEVICE(handle);
+  if (hipMemcpy2DAsync(out->limbs, sizeof(uint32_t) * out->stride, ctx->r2, sizeof(uint32_t) * ctx->r2_stride,
+                       sizeof(uint32_t) * m->count, m->nlimbs, hipMemcpyDeviceToDevice, handle->stream) != hipSuccess ||
+      hipStreamSynchronize(handle->stream) != hipSuccess)
+    return xmpErrorCuda;
+  XMP_CHECK_HIP();
   return xmpErrorSuccess;
 }
 
@@ -224,7 +240,7 @@ xmpError_t XMPAPI xmpTunerRun(xmpHandle_t handle, uint32_t precision, uint32_t e
     // a layout more than twice the size's width only wastes multiply-adds
     if (variant_rbits(v) > 2 * precision + 128) continue;
     uint32_t *r2 = nullptr, r2s = 0;
-    if ((e = montgomery_prepare(handle, M, v, &r2, &r2s)) != xmpErrorSuccess) break;
+    if ((e = montgomery_prepare(handle, M, vi, &r2, &r2s)) != xmpErrorSuccess) break;
     for (uint32_t w = (w0 > 2 ? w0 - 1 : 1); w <= w0 + 1 && w <= 8; w++) {
       size_t tw = powm_table_words(v, round_up(count, kStrideAlign), w);
       uint32_t *table = (uint32_t *)handle->da(tw * sizeof(uint32_t));

@@ -247,6 +247,25 @@ struct Mont {
 };
 
 
+// An instance's ND normalised radix-2^D digits, staged in LDS at b, written
+// as `len` interleaved 32-bit limbs; lane l of the group writes limbs l, l+L, ...
+template <int L, int D>
+__device__ __forceinline__ void store_limbs(const uint32_t *b, int ND, uint32_t *out, uint32_t stride, uint32_t inst,
+                                            uint32_t len, int l) {
+  for (uint32_t w = (uint32_t)l; w < len; w += L) {
+    uint32_t bit = 32u * w;
+    uint32_t d0 = bit / D, s0 = bit % D;
+    uint32_t x0 = d0 < (uint32_t)ND ? b[d0] : 0u;
+    uint32_t x1 = d0 + 1 < (uint32_t)ND ? b[d0 + 1] : 0u;
+    uint32_t x2 = d0 + 2 < (uint32_t)ND ? b[d0 + 2] : 0u;
+    uint64_t lo = ((uint64_t)x1 << D) | x0;
+    uint32_t v = (uint32_t)(lo >> s0);
+    uint32_t sh2 = 2u * D - s0;
+    if (sh2 < 32) v |= x2 << sh2;
+    out[(size_t)w * stride + inst] = v;
+  }
+}
+
 __device__ __forceinline__ uint32_t exp_bits(const Operand &e, uint32_t inst, uint32_t pos, uint32_t len) {
   uint32_t w = pos >> 5, s = pos & 31;
   uint32_t lo = ld_limb(e, inst, w);
@@ -368,21 +387,7 @@ __device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, u
 #pragma unroll
   for (int k = 0; k < K; k++) b[l * K + k] = (ne && odd) ? r[k] : 0u;
   wave_sync();
-  if (active) {
-    const uint32_t oi = out_index(args.out, gi);
-    for (uint32_t w = (uint32_t)l; w < args.out.len; w += L) {
-      uint32_t bit = 32u * w;
-      uint32_t d0 = bit / D, s0 = bit % D;
-      uint32_t x0 = d0 < (uint32_t)ND ? b[d0] : 0u;
-      uint32_t x1 = d0 + 1 < (uint32_t)ND ? b[d0 + 1] : 0u;
-      uint32_t x2 = d0 + 2 < (uint32_t)ND ? b[d0 + 2] : 0u;
-      uint64_t lo = ((uint64_t)x1 << D) | x0;
-      uint32_t v = (uint32_t)(lo >> s0);
-      uint32_t sh2 = 2u * D - s0;
-      if (sh2 < 32) v |= x2 << sh2;
-      args.out.data[(size_t)w * args.out.stride + oi] = v;
-    }
-  }
+  if (active) store_limbs<L, D>(b, ND, args.out.data, args.out.stride, out_index(args.out, gi), args.out.len, l);
 }
 
 template <int L, int K, int D>
@@ -433,6 +438,95 @@ const VariantEntry kVariants[] = {
 };
 #undef XMP_VARIANT
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+// ---- R^2 mod m for long moduli by Montgomery squarings ------------------
+// With R = 2^rbits = 2^(t * 2^j) (t odd), Montgomery squaring maps
+// x*R mod m to x^2*R mod m, so j squarings of Y = 2^t*R mod m give
+// 2^rbits*R = R^2 mod m.  Y is one division of 2^(rbits+t), whose quotient
+// has only about t + (rbits - precision) bits: for 8192-bit moduli 17 limbs
+// instead of the 257 of a direct 2^(2 rbits) mod m.
+struct R2Args {
+  Operand mod;
+  const uint32_t *y;   // 2^(rbits+t) mod m, mod.len limbs, interleaved by modulus
+  uint32_t y_stride;
+  uint32_t *out;       // the squared value (< 2m), out_len = mod.len + 1 limbs
+  uint32_t out_stride, out_len;
+  uint32_t count, squarings;
+};
+
+template <int L, int K, int D>
+__global__ void __launch_bounds__(64, 2) r2_square_kernel(R2Args args) {
+  using M = Mont<L, K, D>;
+  constexpr int ND = M::ND;
+  constexpr int NDP = (ND + 3) / 4 * 4;
+  constexpr int G = 64 / L;
+  __shared__ __attribute__((aligned(16))) uint32_t bbuf[lds_words<L, K, D>()];
+  const int lane = threadIdx.x;
+  const int grp = lane / L;
+  M mont(lane);
+  const int l = mont.g.l;
+  const uint32_t local = blockIdx.x * G + grp;
+  const bool active = local < args.count;
+  const uint32_t mi = active ? local : args.count - 1;
+  uint32_t *b = bbuf + grp * NDP;
+  uint32_t a[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const uint32_t d = (uint32_t)(l * K + k);
+    mont.m[k] = limb_digit_guarded<D>(args.mod.data, args.mod.stride, mi, args.mod.len, d);
+    a[k] = limb_digit_guarded<D>(args.y, args.y_stride, mi, args.mod.len, d);
+  }
+  {
+    const uint32_t m0 = ld_limb(args.mod, mi, 0);
+    uint32_t inv = m0;
+#pragma unroll
+    for (int it = 0; it < 4; it++) inv = inv * (2u - m0 * inv);
+    mont.np0 = (0u - inv) & M::MASK;
+  }
+  for (uint32_t s = 0; s < args.squarings; s++) {
+#pragma unroll
+    for (int k = 0; k < K; k++) b[l * K + k] = a[k];
+    wave_sync();
+    mont.sqr_group(a, a, b);
+    wave_sync();
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) b[l * K + k] = a[k];
+  wave_sync();
+  if (active) store_limbs<L, D>(b, ND, args.out, args.out_stride, mi, args.out_len, l);
+}
+
+// r2 = v - m if v >= m else v, one lane per modulus (v < 2m has len+1 limbs)
+__global__ void __launch_bounds__(256) r2_canonical_kernel(uint32_t *r2, uint32_t r2_stride, const uint32_t *v,
+                                                           uint32_t v_stride, const uint32_t *m, uint32_t m_len,
+                                                           uint32_t m_stride, uint32_t count) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  uint32_t br = 0;
+  for (uint32_t k = 0; k <= m_len; k++) {
+    const uint32_t mk = k < m_len ? m[(size_t)k * m_stride + t] : 0u;
+    (void)__builtin_subc(v[(size_t)k * v_stride + t], mk, br, &br);
+  }
+  const bool keep = br != 0;   // v < m
+  br = 0;
+  for (uint32_t k = 0; k < m_len; k++) {
+    const uint32_t vk = v[(size_t)k * v_stride + t];
+    const uint32_t d = __builtin_subc(vk, m[(size_t)k * m_stride + t], br, &br);
+    r2[(size_t)k * r2_stride + t] = keep ? vk : d;
+  }
+}
+
+struct R2Entry {
+  PowmVariant v;
+  void (*kernel)(R2Args);
+};
+#define XMP_R2(L, K, D) {{L, K, D}, r2_square_kernel<L, K, D>}
+// the layouts whose moduli can exceed the register division (more than
+// kMaxRegDivisorLimbs limbs)
+const R2Entry kR2Variants[] = {
+    XMP_R2(8, 29, 27), XMP_R2(8, 38, 27), XMP_R2(16, 19, 27), XMP_R2(32, 20, 26), XMP_R2(64, 20, 26),
+};
+#undef XMP_R2
 
 // ---- persistent mixed-size kernel --------------------------------------
 // Every wave pulls (class, tile) pairs from a device queue ordered by
@@ -553,6 +647,39 @@ hipError_t launch_powm_variant(int variant, const PowmArgs &args, hipStream_t st
   const PowmVariant &v = kVariants[variant].v;
   const uint32_t per_block = 64u / (uint32_t)v.lanes;
   hipLaunchKernelGGL(kVariants[variant].kernel, dim3(div_round_up(args.count, per_block)), dim3(64), 0, stream, args);
+  return hipGetLastError();
+}
+
+hipError_t launch_montgomery_r2(int variant, uint32_t *r2, uint32_t r2_stride, const uint32_t *m, uint32_t m_len,
+                                uint32_t m_stride, uint32_t count, uint32_t *ws, hipStream_t stream) {
+  if (variant < 0 || variant >= kNumVariants || count == 0) return hipErrorInvalidValue;
+  const PowmVariant &v = kVariants[variant].v;
+  const uint32_t rbits = variant_rbits(v);
+  if (m_len <= kMaxRegDivisorLimbs) return launch_pow2_mod(r2, m_len, r2_stride, m, m_len, m_stride, count, 2 * rbits,
+                                                           nullptr, stream);
+  void (*kernel)(R2Args) = nullptr;
+  for (const R2Entry &e : kR2Variants)
+    if (e.v.lanes == v.lanes && e.v.digits == v.digits && e.v.radix == v.radix) kernel = e.kernel;
+  if (!kernel || !ws) return hipErrorInvalidValue;
+  const uint32_t j = (uint32_t)__builtin_ctz(rbits), t = rbits >> j;
+  // Y = 2^(rbits+t) mod m into r2, then j squarings into the workspace (whose
+  // division use is over), then the canonical value back into r2
+  hipError_t e = launch_pow2_mod(r2, m_len, r2_stride, m, m_len, m_stride, count, rbits + t, ws, stream);
+  if (e != hipSuccess) return e;
+  const uint32_t ws_stride = round_up(count, 256);
+  R2Args a;
+  a.mod = Operand{m, m_len, m_stride, count, nullptr, 0};
+  a.y = r2;
+  a.y_stride = r2_stride;
+  a.out = ws;
+  a.out_stride = ws_stride;
+  a.out_len = m_len + 1;
+  a.count = count;
+  a.squarings = j;
+  hipLaunchKernelGGL(kernel, dim3(div_round_up(count, 64u / (uint32_t)v.lanes)), dim3(64), 0, stream, a);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(r2_canonical_kernel, dim3(div_round_up(count, 256)), dim3(256), 0, stream, r2, r2_stride, ws,
+                     ws_stride, m, m_len, m_stride, count);
   return hipGetLastError();
 }
 

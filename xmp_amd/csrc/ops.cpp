@@ -257,8 +257,8 @@ xmpError_t XMPAPI xmpIntegersPowmAsync(xmpHandle_t handle, xmpIntegers_t out, co
   uint32_t *r2 = (uint32_t *)handle->scratch;
   uint32_t *divws = r2 + r2_words;
   uint32_t *table = divws + div_words;
-  if (launch_pow2_mod(r2, mod->nlimbs, mstride, mod->limbs, mod->nlimbs, mod->stride, mod->count,
-                      2 * variant_rbits(v), div_words ? divws : nullptr, handle->stream) != hipSuccess)
+  if (launch_montgomery_r2(c.variant, r2, mstride, mod->limbs, mod->nlimbs, mod->stride, mod->count,
+                           div_words ? divws : nullptr, handle->stream) != hipSuccess)
     return xmpErrorCuda;
   PowmArgs args;
   args.out = make_out(out, obuf, p, 0);

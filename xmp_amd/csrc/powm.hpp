@@ -51,6 +51,11 @@ inline uint32_t variant_rbits(const PowmVariant &v) { return (uint32_t)(v.lanes 
 size_t powm_table_words(const PowmVariant &v, uint32_t count, uint32_t window_bits);
 // launch the exponentiation kernel for one variant
 hipError_t launch_powm_variant(int variant, const PowmArgs &args, hipStream_t stream);
+// R^2 mod m (R = 2^rbits of the variant) for m[:, t], t < count, canonical, into
+// r2 (m_len limbs, interleaved with r2_stride).  Moduli longer than
+// kMaxRegDivisorLimbs need `ws` of divmod_scratch_words(m_len, count) words.
+hipError_t launch_montgomery_r2(int variant, uint32_t *r2, uint32_t r2_stride, const uint32_t *m, uint32_t m_len,
+                                uint32_t m_stride, uint32_t count, uint32_t *ws, hipStream_t stream);
 // variants usable for a modulus precision (in bits): those with R >= 2^(precision+2)
 bool variant_fits(const PowmVariant &v, uint32_t precision);
 // whether the persistent mixed-size kernel runs this variant

@@ -35,7 +35,7 @@
 #include "tuner.hpp"
 
 namespace xmp {
-xmpError_t montgomery_prepare(xmpHandle_t h, xmpIntegers_t mod, const PowmVariant &v, uint32_t **r2_out,
+xmpError_t montgomery_prepare(xmpHandle_t h, xmpIntegers_t mod, int variant, uint32_t **r2_out,
                               uint32_t *stride_out);
 }
 
@@ -432,7 +432,7 @@ xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const
           ly.r2 = o.r2;
           ly.r2_stride = o.r2_stride;
         }
-      if (!ly.r2 && (e = montgomery_prepare(handle, mods, powm_variant(ch.variant), &ly.r2, &ly.r2_stride)))
+      if (!ly.r2 && (e = montgomery_prepare(handle, mods, ch.variant, &ly.r2, &ly.r2_stride)))
         return fail(e);
       set.push_back(ly);
       const PowmVariant v = powm_variant(ch.variant);

@@ -12,6 +12,7 @@ def declare(L):
         "xmpMontgomeryDestroy": [_vp, _vp],
         "xmpMontgomeryGetLayout": [_vp, _vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32), ctypes.POINTER(_u32),
                                    ctypes.POINTER(_u32)],
+        "xmpMontgomeryGetR2": [_vp, _vp, _vp],
         "xmpIntegersPowmMontgomeryAsync": [_vp, _vp, _vp, _vp, _vp, _u32],
         "xmpBatchCreate": [_vp, ctypes.POINTER(_vp), _u32, _vp, _vp, _vp, _vp, _vp],
         "xmpBatchDestroy": [_vp, _vp],
@@ -45,6 +46,16 @@ class Montgomery:
         v = [_u32() for _ in range(4)]
         check(lib().xmpMontgomeryGetLayout(self.handle.h, self.ctx, *[ctypes.byref(x) for x in v]), "GetLayout")
         return tuple(x.value for x in v)
+
+    def r2(self):
+        """The resident R^2 mod m as a (count, limbs) array."""
+        from . import check, lib
+        out = self.handle.integers(self.mod.precision, self.mod.count)
+        try:
+            check(lib().xmpMontgomeryGetR2(self.handle.h, self.ctx, out.x), "xmpMontgomeryGetR2")
+            return out.get()
+        finally:
+            out.close()
 
     def powm_async(self, out, base, exp, count):
         from . import check, lib
