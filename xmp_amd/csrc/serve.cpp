@@ -10,7 +10,8 @@
 // Keys stay resident on the device with their Montgomery constants.  A
 // scheduler thread takes whatever requests have arrived, admits as many as
 // the HBM budget allows into a free batch slot, and launches that batch on
-// the slot's own stream; slots run concurrently, so requests arriving while
+// the slot's own pair of streams (one per request kind); slots run
+// concurrently, so requests arriving while
 // others are on the GPU are admitted as soon as a slot and memory free up and
 // their kernels overlap with the ones already running.  Requests are
 //   kind 0: out = base^exp mod n_k                         (2048-bit)
