@@ -146,6 +146,26 @@ def test_powm_fermat_and_crt_sample_key(handle):
         assert O.to_int(got[i]) == want
 
 
+@pytest.mark.parametrize("nb,ne,nm", [(1, 1, 1), (None, 1, 1), (None, 7, 7), (1, 7, None), (None, None, 3),
+                                      (2, None, None)])
+def test_powm_broadcast_patterns(handle, nb, ne, nm):
+    """The reference's instantiation patterns (test.cu:2393-2910): inputs with
+    fewer instances than the output are read at i % count ((N,N,1,1),
+    (N,N,M,M), (N,1,M,N), ...), and longer inputs are read from the start."""
+    n, bits = 300, 1024
+    limbs = bits // 32
+    cb, ce, cm = (c if c is not None else 2 * n for c in (nb, ne, nm))
+    m = inputs.odd_moduli(1300 + cm, cm, limbs)
+    b = inputs.uniform(1301 + cb, cb, limbs)
+    e = inputs.uniform(1302 + ce, ce, limbs)
+    M, B, E = handle.from_array(m), handle.from_array(b), handle.from_array(e)
+    Out = handle.integers(bits, n)
+    handle.powm(Out, B, E, M, n)
+    i = np.arange(n)
+    want = gmp(O.GO_POWM, n, limbs, b[i % cb], e[i % ce], m[i % cm])
+    assert_rows_equal(Out.get(), want, "powm with base/exp/mod counts %d/%d/%d" % (cb, ce, cm))
+
+
 @pytest.mark.parametrize("bits,n", [(2048, 64), (4096, 64), (6144, 32), (8192, 32), (16384, 8), (32768, 4)])
 def test_montgomery_r2_canonical(handle, bits, n):
     """The resident R^2 mod m of a context is canonical (< m) and exact: one
