@@ -291,3 +291,59 @@ def test_error_codes(handle):
     hip.device_sync()
     for x in (a, b, c128, c, bb, e, m):
         x.close()
+
+
+def test_policy_parameters_and_scratch_limit(fresh_handle):
+    """xmpExecutionPolicySet/GetParameter (reference xmp.cu:283-314), and the
+    scratch limit as the reference's failure handling uses it
+    (powm_operators.h:308-328): a tight limit shrinks the launch (smaller
+    window, chunks of instances) without changing any result, and a limit
+    below the smallest plan gives xmpErrorIncreaseScratchLimit."""
+    h = fresh_handle
+    p = h.policy()
+    v = ctypes.c_uint64(99)
+    assert L().xmpExecutionPolicyGetParameter(h.h, p, xmp_amd.PARAM_ALGORITHM, ctypes.byref(v)) == 0
+    assert v.value == xmp_amd.ALG_DEFAULT
+    assert L().xmpExecutionPolicySetParameter(h.h, p, xmp_amd.PARAM_ALGORITHM, xmp_amd.ALG_DISTRIBUTEDMP) == 0
+    assert L().xmpExecutionPolicyGetParameter(h.h, p, xmp_amd.PARAM_ALGORITHM, ctypes.byref(v)) == 0
+    assert v.value == xmp_amd.ALG_DISTRIBUTEDMP
+    assert L().xmpExecutionPolicySetParameter(h.h, p, xmp_amd.PARAM_ALGORITHM, 7) == xmp_amd.INVALID_PARAMETER
+    assert L().xmpExecutionPolicySetParameter(h.h, p, 5, 0) == xmp_amd.INVALID_PARAMETER
+    h.set_algorithm(xmp_amd.ALG_DEFAULT)
+    h.set_scratch_limit(1 << 20)
+    assert L().xmpExecutionPolicyGetParameter(h.h, p, xmp_amd.PARAM_SCRATCH_LIMIT, ctypes.byref(v)) == 0
+    assert v.value == 1 << 20
+
+    # 3,000 x 2048-bit: the full window table (~57 MB) does not fit 10 MB
+    n, limbs = 3000, 64
+    m = inputs.odd_moduli(61, n, limbs)
+    b = inputs.uniform(62, n, limbs)
+    e = inputs.uniform(63, n, limbs)
+    M, B, E = h.from_array(m), h.from_array(b), h.from_array(e)
+    Out = h.integers(2048, n)
+    want = O.gmp_batch(O.GO_POWM, n, limbs, b, e, m, threads=8)[0]
+    h.set_scratch_limit(10 << 20)
+    h.powm(Out, B, E, M)
+    assert_rows_equal(Out.get(), want, "powm under a 10 MB scratch limit")
+    h.set_scratch_limit(1000)
+    with pytest.raises(xmp_amd.XmpError) as err:
+        h.powm(Out, B, E, M)
+    assert err.value.code == xmp_amd.INCREASE_SCRATCH_LIMIT
+    h.set_scratch_limit(0)   # the limit covers every op's scratch, imports included
+    for x in (M, B, E, Out):
+        x.close()
+
+    # 140,000 x 512-bit under 24 MB: the instances run in chunks
+    n, limbs = 140000, 16
+    m = inputs.odd_moduli(64, n, limbs)
+    b = inputs.uniform(65, n, limbs)
+    e = inputs.uniform(66, n, limbs)
+    M, B, E = h.from_array(m), h.from_array(b), h.from_array(e)
+    Out = h.integers(512, n)
+    h.set_scratch_limit(24 << 20)   # above the 9 MB of R^2 mod m, below one plain table
+    h.powm(Out, B, E, M)
+    h.set_scratch_limit(0)
+    want = O.gmp_batch(O.GO_POWM, n, limbs, b, e, m, threads=16)[0]
+    assert_rows_equal(Out.get(), want, "chunked powm under a 24 MB scratch limit")
+    for x in (M, B, E, Out):
+        x.close()
