@@ -97,3 +97,16 @@ def test_canonical_work_formula():
 def test_bench_refuses_several_gpus():
     r = subprocess.run(["python3", os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_headers_compile_for_c_and_cxx_callers():
+    """include/xmp.h and xmp_ext.h compile as C11 (tests/cxx/header_c.c) and
+    as C++17 in a whole caller program (tests/cxx/crt_decrypt.cpp, which the
+    GPU tests run): the drop-in boundary is usable from both languages."""
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(rocm, "include"), "-D__HIP_PLATFORM_AMD__"]
+    cxx = os.path.join(ROOT, "tests", "cxx")
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-fsyntax-only", *inc, os.path.join(cxx, "header_c.c")],
+                   check=True)
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", *inc,
+                    os.path.join(cxx, "crt_decrypt.cpp")], check=True)

@@ -347,3 +347,15 @@ def test_policy_parameters_and_scratch_limit(fresh_handle):
     assert_rows_equal(Out.get(), want, "chunked powm under a 24 MB scratch limit")
     for x in (M, B, E, Out):
         x.close()
+
+
+def test_cxx_caller_crt_decrypt():
+    """A C++ program built against include/xmp.h and linked to the library
+    (tests/cxx/crt_decrypt.cpp, compiled by __graft_entry__.build()): RSA
+    encryption and CRT decryption of 4,096 messages round-trip exactly."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cxx", "_build", "crt_decrypt")
+    assert os.path.exists(exe), "tests/cxx/_build/crt_decrypt is not built (run __graft_entry__.build())"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "crt ok" in r.stdout, r.stdout + r.stderr
