@@ -24,7 +24,11 @@ def _class_inputs(seed, bits, n):
     return m, b, e
 
 
-def test_batch_mixed_sizes_replays(handle):
+@pytest.mark.parametrize("classes", [CLASSES, [(2048, 300)], [(8192, 20)]], ids=["mixed", "one-2048", "one-8192"])
+def test_batch_replays(handle, classes):
+    """Mixed classes run in the persistent kernel; a one-class batch runs
+    that class's own kernel in the captured graph."""
+    CLASSES = classes
     arrays, host = [], []
     for k, (bits, n) in enumerate(CLASSES):
         m, b, e = _class_inputs(1000 + k, bits, n)

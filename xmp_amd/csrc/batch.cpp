@@ -8,7 +8,8 @@
 // one device queue ordered by decreasing modelled cost (longest tiles first),
 // so short tiles fill the SIMDs that long ones leave idle.  The queue reset
 // and the persistent kernel are captured in a HIP graph on the first launch
-// and replayed afterwards.
+// and replayed afterwards.  A one-class batch captures that class's own
+// kernel instead: with nothing to interleave, the queue only costs time.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -36,6 +37,7 @@ struct _xmpBatch_t {
   std::vector<uint32_t *> r2, table;
   std::vector<uint32_t> r2_stride;
   PowmArgs *d_classes = nullptr;
+  PowmArgs single;   // a one-class batch's arguments (that class's kernel runs directly)
   int32_t *d_variant = nullptr;
   uint32_t *d_tile_class = nullptr, *d_tile_index = nullptr, *d_counter = nullptr;
   uint32_t ntiles = 0, grid = 0;
@@ -139,6 +141,7 @@ xmpError_t XMPAPI xmpBatchCreate(xmpHandle_t handle, xmpBatch_t *batch, uint32_t
     const double cost = tile_cost(v, exp[c]->precision, ch.window_bits);
     for (uint32_t t = 0; t < nt; t++) tiles.push_back({c, t, cost});
   }
+  if (e == xmpErrorSuccess && classes == 1) b->single = args[0];
   if (e == xmpErrorSuccess) {
     // longest tiles first (LPT order); stable so each class's tiles stay in order
     std::stable_sort(tiles.begin(), tiles.end(), [](const Tile &x, const Tile &y) { return x.cost > y.cost; });
@@ -217,7 +220,11 @@ xmpError_t XMPAPI xmpBatchLaunchAsync(xmpHandle_t handle, xmpBatch_t batch) {
     if (!batch->capture_stream && hipStreamCreate(&batch->capture_stream) != hipSuccess) return xmpErrorCuda;
     if (hipStreamBeginCapture(batch->capture_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
       return xmpErrorCuda;
-    hipError_t st = launch_powm_batch(ba, batch->grid, batch->lds_bytes, batch->capture_stream);
+    // one class has no tiles of other classes to interleave: its own kernel
+    // runs without the queue (and without the persistent kernel's calls)
+    hipError_t st = batch->classes == 1
+                        ? launch_powm_variant(batch->variant[0], batch->single, batch->capture_stream)
+                        : launch_powm_batch(ba, batch->grid, batch->lds_bytes, batch->capture_stream);
     hipGraph_t g = nullptr;
     hipError_t st2 = hipStreamEndCapture(batch->capture_stream, &g);
     if (st != hipSuccess || st2 != hipSuccess || !g) return xmpErrorCuda;
