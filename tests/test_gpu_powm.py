@@ -189,6 +189,48 @@ def test_montgomery_r2_canonical(handle, bits, n):
     assert_rows_equal(got, want, "R^2 mod m, %d-bit moduli, R = 2^%d" % (bits, rbits))
 
 
+def test_montgomery_context_indices_and_in_place(fresh_handle):
+    """xmpIntegersPowmMontgomeryAsync with policy slots (0 scatter, 1/2 base and
+    exponent gather, 3 the modulus inside the context) and in place (out ==
+    base), against GMP; the same index semantics as xmpIntegersPowm."""
+    from xmp_amd import hip
+    from xmp_amd.ext import Montgomery
+    h = fresh_handle
+    n, nm, limbs = 256, 16, 64
+    m = inputs.odd_moduli(1701, nm, limbs)
+    b = inputs.uniform(1702, n, limbs)
+    e = inputs.uniform(1703, 8, limbs)
+    rng = np.random.default_rng(17)
+    im = rng.integers(0, 10 * nm, 48).astype(np.uint32)
+    ie = rng.integers(0, 8, 24).astype(np.uint32)
+    io = rng.permutation(n).astype(np.uint32)
+    M, B, E = h.from_array(m), h.from_array(b), h.from_array(e)
+    Out = h.integers(2048, n)
+    ctx = Montgomery(h, M, 2048, n)
+    try:
+        h.set_indices(0, io)
+        h.set_indices(2, ie)
+        h.set_indices(3, im)
+        try:
+            ctx.powm_async(Out, B, E, n)
+            hip.device_sync()
+        finally:
+            h.clear_indices()
+        rows = gmp(O.GO_POWM, n, limbs, b, e[ie[np.arange(n) % 24] % 8], m[im[np.arange(n) % 48] % nm])
+        want = np.zeros_like(rows)
+        want[io] = rows
+        assert_rows_equal(Out.get(), want, "indexed powm through a Montgomery context")
+        # in place: B <- B^E mod M[i % nm]
+        ctx.powm_async(B, B, E, n)
+        hip.device_sync()
+        want = gmp(O.GO_POWM, n, limbs, b, e[np.arange(n) % 8], m[np.arange(n) % nm])
+        assert_rows_equal(B.get(), want, "in-place powm through a Montgomery context")
+    finally:
+        ctx.close()
+        for x in (M, B, E, Out):
+            x.close()
+
+
 def test_powm_indices_and_resident_moduli(handle):
     """Policy index gather (slots 1..3) and scatter (slot 0), as the reference
     powm kernels read them (powm_implementations.h:417-418, copy-out :480-516)."""
