@@ -276,3 +276,11 @@ def test_powm_config1_full(handle):
     n = 4096
     got, b, e, m = _powm_case(handle, 1001, n, 1024)
     assert_rows_equal(got, gmp(O.GO_POWM, n, 32, b, e, m), "config 1")
+
+
+def test_powm_2048_full_size_every_row(handle):
+    """The headline shape, 65,536 x 2048-bit, every row against GMP (about
+    7 s of GMP on 16 host threads); the bench checks only a sample."""
+    n = 65536
+    got, b, e, m = _powm_case(handle, 2048, n, 2048)
+    assert_rows_equal(got, gmp(O.GO_POWM, n, 64, b, e, m), "65,536 x 2048-bit powm")
