@@ -187,9 +187,9 @@ xmpError_t XMPAPI xmpIntegersPowm(xmpHandle_t handle, xmpIntegers_t out, const x
 xmpError_t XMPAPI xmpIntegersPowmAsync(xmpHandle_t handle, xmpIntegers_t out, const xmpIntegers_t base,
                                        const xmpIntegers_t exp, const xmpIntegers_t mod, uint32_t count);
 
-/* ---- bitwise: reference xmp.h:206-226.  Outside this library's scope:
- * the entry points exist so that callers link, and return
- * xmpErrorUnsupported. ---- */
+/* ---- bitwise: reference xmp.h:206-226, src/logic.cu.  Index slots follow
+ * the argument order (0 = output, then the inputs); Cmp and Popc write one
+ * int32/uint32 per instance to a host or device array. ---- */
 xmpError_t XMPAPI xmpIntegersShf(xmpHandle_t handle, xmpIntegers_t c, const xmpIntegers_t a, const int32_t *shift,
                                  const uint32_t shift_count, uint32_t count);
 xmpError_t XMPAPI xmpIntegersShfAsync(xmpHandle_t handle, xmpIntegers_t c, const xmpIntegers_t a,

@@ -62,6 +62,40 @@ def test_powm_above_8192_bits_special_values(handle):
     assert_rows_equal(Out.get(), gmp(O.GO_POWM, n, limbs, b, e, m), "special values above 8192 bits")
 
 
+@pytest.mark.parametrize("bits,exp_bits,n", [(34816, 96, 2), (40960, 1024, 2), (65536, 64, 2), (131072, 32, 1)])
+def test_powm_wider_than_lane_layouts(handle, bits, exp_bits, n):
+    """Moduli above 33,278 bits, beyond every lane layout: one workgroup per
+    instance (8, 16 and 32 digits per thread), R^2 mod m from a short division
+    and Montgomery squarings; against GMP."""
+    got, b, e, m = _powm_case(handle, 3100 + bits + exp_bits, n, bits, exp_bits=exp_bits, reduce_base=False)
+    assert_rows_equal(got, gmp(O.GO_POWM, n, bits // 32, b, e, m), "powm %d-bit, %d-bit exponent" % (bits, exp_bits))
+
+
+def test_powm_wider_than_lane_layouts_special_values(handle):
+    """e = 0, b = 0, b = m - 1 with e = 2, a small modulus and an even modulus
+    (defined as 0) in a 34,816-bit container."""
+    limbs, n = 1088, 5
+    m = inputs.odd_moduli(3201, n, limbs)
+    b = inputs.uniform(3202, n, limbs)
+    e = inputs.uniform(3203, n, 2)
+    e[0] = 0
+    b[1] = 0
+    b[2] = m[2]
+    b[2, 0] -= 1
+    e[2] = 0
+    e[2, 0] = 2
+    m[3] = 0
+    m[3, 0] = 0xFFFFFFFB   # 2^32 - 5, prime
+    m[4, 0] &= 0xFFFFFFFE  # even
+    M, B, E = handle.from_array(m), handle.from_array(b), handle.from_array(e)
+    Out = handle.integers(limbs * 32, n)
+    handle.powm(Out, B, E, M)
+    got = Out.get()
+    want = gmp(O.GO_POWM, n, limbs, b, e, m)
+    want[4] = 0
+    assert_rows_equal(got, want, "special values above 33,278 bits")
+
+
 @pytest.mark.parametrize("bits", [1024, 2048])
 def test_powm_matches_c_oracle(handle, bits):
     n = 16

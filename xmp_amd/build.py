@@ -31,6 +31,7 @@ SOURCES = [
     "k_div.hip",
     "k_logic.hip",
     "k_powm.hip",
+    "k_powm_wide.hip",
     "k_probe.hip",
 ]
 

@@ -650,6 +650,14 @@ hipError_t launch_powm_variant(int variant, const PowmArgs &args, hipStream_t st
   return hipGetLastError();
 }
 
+hipError_t launch_r2_canonical(uint32_t *r2, uint32_t r2_stride, const uint32_t *v, uint32_t v_stride,
+                               const uint32_t *m, uint32_t m_len, uint32_t m_stride, uint32_t count,
+                               hipStream_t stream) {
+  hipLaunchKernelGGL(r2_canonical_kernel, dim3(div_round_up(count, 256)), dim3(256), 0, stream, r2, r2_stride, v,
+                     v_stride, m, m_len, m_stride, count);
+  return hipGetLastError();
+}
+
 hipError_t launch_montgomery_r2(int variant, uint32_t *r2, uint32_t r2_stride, const uint32_t *m, uint32_t m_len,
                                 uint32_t m_stride, uint32_t count, uint32_t *ws, hipStream_t stream) {
   if (variant < 0 || variant >= kNumVariants || count == 0) return hipErrorInvalidValue;
@@ -678,9 +686,7 @@ hipError_t launch_montgomery_r2(int variant, uint32_t *r2, uint32_t r2_stride, c
   a.squarings = j;
   hipLaunchKernelGGL(kernel, dim3(div_round_up(count, 64u / (uint32_t)v.lanes)), dim3(64), 0, stream, a);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(r2_canonical_kernel, dim3(div_round_up(count, 256)), dim3(256), 0, stream, r2, r2_stride, ws,
-                     ws_stride, m, m_len, m_stride, count);
-  return hipGetLastError();
+  return launch_r2_canonical(r2, r2_stride, ws, ws_stride, m, m_len, m_stride, count, stream);
 }
 
 }  // namespace xmp

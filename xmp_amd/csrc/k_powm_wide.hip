@@ -1,0 +1,287 @@
+// k_powm_wide.hip -- modular exponentiation for moduli wider than any lane
+// layout holds (above 33,278 bits): one 256-thread workgroup per instance.
+//
+// Reference: the digit-serial fallback the reference uses above 8192 bits
+// (src/include/powm/digitized.h, powm_implementations.h:154-286), with the
+// same fixed-window schedule as every other powm here (powm/powm.h:94-98).
+//
+// The instance's ND = 256*C radix-2^24 digits are spread over the four waves
+// of a workgroup, C per thread, with the CIOS accumulator as 64-bit lazy sums
+// in registers (2*ND products of < 2^48 plus carries stay below 2^64 for
+// ND <= 8192).  Per CIOS step every thread reads the step's multiplier digit
+// from LDS, thread 0 publishes the quotient digit of the next step, and every
+// thread hands the low digit of its retired accumulator to the thread below,
+// through LDS with one workgroup barrier per step (the quotient and the
+// hand-over are double-buffered, so a step's reads never meet the next step's
+// writes).  R = 2^(24*ND) = 3 * 2^(11 + log2 C), so R^2 mod m comes from one
+// short division of 2^(rbits + 3) and 11 + log2 C Montgomery squarings.
+//
+// The layout trades speed for reach: each step pays a workgroup barrier, and
+// a modulus just above a tier pays the next tier's digit count.  It exists so
+// that no modulus size is left unsupported up to 196,606 bits.
+#include "devmp.hpp"
+#include "lanes.hpp"
+#include "powm.hpp"
+
+namespace xmp {
+
+namespace {
+
+constexpr int kWideD = 24;
+constexpr uint32_t kWideMask = (1u << kWideD) - 1;
+constexpr int kWideThreads = 256;
+
+template <int C>
+struct Wide {
+  static constexpr int ND = kWideThreads * C;
+  static_assert(C >= 2, "a thread's lowest accumulator must not depend on its neighbour");
+  static_assert(2.0 * ND * (double)kWideMask * kWideMask + 4.0 * (1ull << 40) < 18446744073709551615.0,
+                "lazy accumulator bound");
+  uint32_t m[C];
+  uint32_t np0;
+  int tid;
+  uint32_t *b;      // ND multiplier digits (LDS)
+  uint32_t *xchg;   // 2 x kWideThreads hand-over digits (LDS)
+  uint32_t *qbuf;   // 2 quotient digits (LDS)
+  uint64_t *cbuf;   // kWideThreads carries (LDS)
+
+  // r = a * b / R (mod m), r < 2m, normalised; b is the LDS multiplier.
+  // Every thread of the workgroup calls it (it contains barriers).
+  __device__ void mul(uint32_t (&r)[C], const uint32_t (&a)[C]) const {
+    uint64_t t[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) t[k] = 0;
+    if (tid == 0) qbuf[0] = ((uint32_t)((uint64_t)a[0] * b[0]) * np0) & kWideMask;
+    __syncthreads();
+    for (int i = 0; i < ND; i++) {
+      const uint32_t bi = b[i];
+      const uint32_t q = qbuf[i & 1];
+      uint64_t f = t[0] + (uint64_t)a[0] * bi + (uint64_t)q * m[0];
+#pragma unroll
+      for (int k = 1; k < C; k++) t[k - 1] = t[k] + (uint64_t)a[k] * bi + (uint64_t)q * m[k];
+      t[0] += f >> kWideD;
+      xchg[(i & 1) * kWideThreads + tid] = (uint32_t)f & kWideMask;
+      // the next quotient digit: thread 0's new lowest accumulator does not
+      // depend on the value it receives from thread 1 (C >= 2)
+      if (tid == 0 && i + 1 < ND) qbuf[(i + 1) & 1] = ((uint32_t)(t[0] + (uint64_t)a[0] * b[i + 1]) * np0) & kWideMask;
+      __syncthreads();
+      t[C - 1] = tid + 1 < kWideThreads ? xchg[(i & 1) * kWideThreads + tid + 1] : 0u;
+    }
+    normalize(r, t);
+  }
+
+  // digits < 2^24: a local carry pass, then carries move up one thread per
+  // round until no thread has one (the value is < 2m < R, so nothing leaves
+  // the top thread; its carry is dropped to bound the loop).
+  __device__ void normalize(uint32_t (&r)[C], const uint64_t (&t)[C]) const {
+    uint64_t c = 0;
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      const uint64_t x = t[k] + c;
+      r[k] = (uint32_t)x & kWideMask;
+      c = x >> kWideD;
+    }
+    for (int round = 0; round < kWideThreads; round++) {
+      cbuf[tid] = tid + 1 < kWideThreads ? c : 0ull;
+      __syncthreads();
+      uint64_t cin = tid > 0 ? cbuf[tid - 1] : 0ull;
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < C; k++) {
+        const uint64_t x = (uint64_t)r[k] + cin;
+        r[k] = (uint32_t)x & kWideMask;
+        cin = x >> kWideD;
+      }
+      c = cin;
+      if (!__syncthreads_or(tid + 1 < kWideThreads && c != 0)) break;
+    }
+  }
+
+  __device__ void stage(const uint32_t (&a)[C]) const {
+#pragma unroll
+    for (int k = 0; k < C; k++) b[tid * C + k] = a[k];
+    __syncthreads();
+  }
+};
+
+// Thread t holds digits t*C .. t*C+C-1; tables store digit t*C+k of entry e
+// at ((e * C + k) * 256 + t), so a table access is coalesced over threads.
+template <int C>
+__global__ void __launch_bounds__(kWideThreads) powm_wide_kernel(PowmArgs args, uint32_t squarings) {
+  using W = Wide<C>;
+  constexpr int ND = W::ND;
+  __shared__ uint32_t s_b[ND];
+  __shared__ uint32_t s_x[2 * kWideThreads];
+  __shared__ uint32_t s_q[2];
+  __shared__ uint64_t s_c[kWideThreads];
+  W w;
+  w.tid = threadIdx.x;
+  w.b = s_b;
+  w.xchg = s_x;
+  w.qbuf = s_q;
+  w.cbuf = s_c;
+  const int tid = threadIdx.x;
+  const uint32_t li = blockIdx.x;   // the grid is exactly args.count workgroups
+  const uint32_t gi = args.start + li;
+  const uint32_t mi = in_index(args.mod, gi);
+#pragma unroll
+  for (int k = 0; k < C; k++)
+    w.m[k] = limb_digit_guarded<kWideD>(args.mod.data, args.mod.stride, mi, args.mod.len, (uint32_t)(tid * C + k));
+  {
+    const uint32_t m0 = ld_limb(args.mod, mi, 0);
+    uint32_t inv = m0;
+#pragma unroll
+    for (int it = 0; it < 4; it++) inv = inv * (2u - m0 * inv);
+    w.np0 = (0u - inv) & kWideMask;
+  }
+  uint32_t a[C];
+
+  if (squarings) {
+    // R^2 set-up: args.base holds 2^t R mod m (one per modulus); square it
+    for (int k = 0; k < C; k++)
+      a[k] = limb_digit_guarded<kWideD>(args.base.data, args.base.stride, mi, args.base.len, (uint32_t)(tid * C + k));
+    for (uint32_t s = 0; s < squarings; s++) {
+      w.stage(a);
+      w.mul(a, a);
+      __syncthreads();
+    }
+  } else {
+    const uint32_t bi = in_index(args.base, gi), ei = in_index(args.exp, gi);
+    const uint32_t wbits = args.window_bits;
+    const uint32_t T = 1u << wbits;
+    uint32_t *tab = args.table + (size_t)li * T * ND;
+    auto tab_at = [&](uint32_t e, int k) -> uint32_t * { return tab + ((size_t)e * C + k) * kWideThreads + tid; };
+    // slot 0 = R mod m, slot 1 = base R mod m, slot e = slot e-1 * slot 1
+#pragma unroll
+    for (int k = 0; k < C; k++)
+      s_b[tid * C + k] = limb_digit_guarded<kWideD>(args.r2, args.r2_stride, mi, args.mod.len, (uint32_t)(tid * C + k));
+#pragma unroll
+    for (int k = 0; k < C; k++) a[k] = (tid == 0 && k == 0) ? 1u : 0u;
+    __syncthreads();
+    w.mul(a, a);
+#pragma unroll
+    for (int k = 0; k < C; k++) *tab_at(0, k) = a[k];
+#pragma unroll
+    for (int k = 0; k < C; k++)
+      a[k] = limb_digit_guarded<kWideD>(args.base.data, args.base.stride, bi, args.base.len, (uint32_t)(tid * C + k));
+    w.mul(a, a);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C; k++) *tab_at(1, k) = a[k];
+    w.stage(a);
+    for (uint32_t e = 2; e < T; e++) {
+      w.mul(a, a);
+#pragma unroll
+      for (int k = 0; k < C; k++) *tab_at(e, k) = a[k];
+    }
+    __syncthreads();
+    const uint32_t bits = args.exp.len * 32;
+    uint32_t pos = bits - ((bits % wbits) ? (bits % wbits) : wbits);
+    auto window = [&](uint32_t p, uint32_t len) {
+      const uint32_t wd = p >> 5, sh = p & 31;
+      return funnel(ld_limb(args.exp, ei, wd + 1), ld_limb(args.exp, ei, wd), sh) & ((1u << len) - 1);
+    };
+    uint32_t idx = window(pos, bits - pos);
+#pragma unroll
+    for (int k = 0; k < C; k++) a[k] = *tab_at(idx, k);
+    while (pos > 0) {
+      pos -= wbits;
+      idx = window(pos, wbits);
+      for (uint32_t s = 0; s < wbits; s++) {
+        w.stage(a);
+        w.mul(a, a);
+        __syncthreads();
+      }
+#pragma unroll
+      for (int k = 0; k < C; k++) s_b[tid * C + k] = *tab_at(idx, k);
+      __syncthreads();
+      w.mul(a, a);
+      __syncthreads();
+    }
+    // leave Montgomery form (a * 1 / R <= m), then r == m -> 0, even m -> 0
+#pragma unroll
+    for (int k = 0; k < C; k++) s_b[tid * C + k] = (tid == 0 && k == 0) ? 1u : 0u;
+    __syncthreads();
+    w.mul(a, a);
+    bool ne = false;
+#pragma unroll
+    for (int k = 0; k < C; k++) ne |= a[k] != w.m[k];
+    ne = __syncthreads_or(ne) != 0;
+    const bool odd = (ld_limb(args.mod, mi, 0) & 1u) != 0;
+#pragma unroll
+    for (int k = 0; k < C; k++)
+      if (!(ne && odd)) a[k] = 0;
+  }
+  __syncthreads();
+  w.stage(a);
+  uint32_t *out = args.out.data;
+  const uint32_t oi = squarings ? mi : out_index(args.out, gi);
+  for (uint32_t wd = (uint32_t)tid; wd < args.out.len; wd += kWideThreads) {
+    const uint32_t bit = 32u * wd, d0 = bit / kWideD, s0 = bit % kWideD;
+    const uint32_t x0 = d0 < (uint32_t)ND ? s_b[d0] : 0u;
+    const uint32_t x1 = d0 + 1 < (uint32_t)ND ? s_b[d0 + 1] : 0u;
+    const uint32_t x2 = d0 + 2 < (uint32_t)ND ? s_b[d0 + 2] : 0u;
+    uint64_t lo = ((uint64_t)x1 << kWideD) | x0;
+    uint32_t v = (uint32_t)(lo >> s0);
+    const uint32_t sh2 = 2u * kWideD - s0;
+    if (sh2 < 32) v |= x2 << sh2;
+    out[(size_t)wd * args.out.stride + oi] = v;
+  }
+}
+
+template <int C>
+hipError_t launch_wide(const PowmArgs &args, uint32_t squarings, hipStream_t stream) {
+  hipLaunchKernelGGL(powm_wide_kernel<C>, dim3(args.count), dim3(kWideThreads), 0, stream, args, squarings);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+uint32_t powm_wide_digits_per_thread(uint32_t precision) {
+  for (uint32_t c : {8u, 16u, 32u})
+    if (kWideD * kWideThreads * c >= precision + 2) return c;
+  return 0;
+}
+
+uint32_t powm_wide_rbits(uint32_t c) { return kWideD * kWideThreads * c; }
+
+size_t powm_wide_table_words(uint32_t c, uint32_t count, uint32_t window_bits) {
+  return (size_t)count * ((size_t)1 << window_bits) * kWideThreads * c;
+}
+
+hipError_t launch_powm_wide(uint32_t c, const PowmArgs &args, hipStream_t stream) {
+  if (args.count == 0) return hipErrorInvalidValue;
+  switch (c) {
+    case 8: return launch_wide<8>(args, 0, stream);
+    case 16: return launch_wide<16>(args, 0, stream);
+    case 32: return launch_wide<32>(args, 0, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_wide_r2(uint32_t c, uint32_t *r2, uint32_t r2_stride, const uint32_t *m, uint32_t m_len,
+                          uint32_t m_stride, uint32_t count, uint32_t *ws, hipStream_t stream) {
+  const uint32_t rbits = powm_wide_rbits(c);
+  const uint32_t j = (uint32_t)__builtin_ctz(rbits), t = rbits >> j;
+  if (!ws || count == 0) return hipErrorInvalidValue;
+  // Y = 2^(rbits+t) mod m into r2 (t = 3: a quotient of a few bits), then j
+  // squarings into the workspace, then the canonical value back into r2
+  hipError_t e = launch_pow2_mod(r2, m_len, r2_stride, m, m_len, m_stride, count, rbits + t, ws, stream);
+  if (e != hipSuccess) return e;
+  const uint32_t ws_stride = round_up(count, 256);
+  PowmArgs a{};
+  a.mod = Operand{m, m_len, m_stride, count, nullptr, 0};
+  a.base = Operand{r2, m_len, r2_stride, count, nullptr, 0};
+  a.out = OutOperand{ws, m_len + 1, ws_stride, nullptr};
+  a.count = count;
+  switch (c) {
+    case 8: e = launch_wide<8>(a, j, stream); break;
+    case 16: e = launch_wide<16>(a, j, stream); break;
+    case 32: e = launch_wide<32>(a, j, stream); break;
+    default: return hipErrorInvalidValue;
+  }
+  if (e != hipSuccess) return e;
+  return launch_r2_canonical(r2, r2_stride, ws, ws_stride, m, m_len, m_stride, count, stream);
+}
+
+}  // namespace xmp

@@ -216,8 +216,22 @@ xmpError_t XMPAPI xmpIntegersPowmAsync(xmpHandle_t handle, xmpIntegers_t out, co
   XMP_SET_DEVICE(handle);
   PowmChoice c;
   xmpError_t e = choose_powm(handle, out->precision, exp->precision, count, p->algorithm, &c);
+  // moduli wider than every lane layout take the workgroup layout (under the
+  // default and digit-serial policies, as the reference's DigitMP fallback)
+  uint32_t wide = 0;
+  if (e == xmpErrorUnsupported && (p->algorithm == xmpAlgorithmDefault || p->algorithm == xmpAlgorithmDigitMP)) {
+    wide = powm_wide_digits_per_thread(out->precision);
+    if (wide) {
+      c.variant = -1;
+      c.window_bits = best_window_bits(exp->precision);
+      e = xmpErrorSuccess;
+    }
+  }
   if (e != xmpErrorSuccess) return e;
-  PowmVariant v = powm_variant(c.variant);
+  const PowmVariant v = wide ? PowmVariant{0, 0, 0} : powm_variant(c.variant);
+  auto table_words = [&](uint32_t n, uint32_t w) {
+    return wide ? powm_wide_table_words(wide, n, w) : powm_table_words(v, round_up(n, kStrideAlign), w);
+  };
 
   bool inplace = (out == a || out == exp || out == mod);
   size_t obytes = inplace ? out_bytes(out) : 0;
@@ -241,7 +255,7 @@ xmpError_t XMPAPI xmpIntegersPowmAsync(xmpHandle_t handle, xmpIntegers_t out, co
   // shrink the chunk (and then the window) until the table fits the limit
   // (reference powm_operators.h:308-328 shrinks the window only)
   while (true) {
-    size_t need = fixed + powm_table_words(v, round_up(chunk, kStrideAlign), wbits) * sizeof(uint32_t);
+    size_t need = fixed + table_words(chunk, wbits) * sizeof(uint32_t);
     if (need <= limit) break;
     if (chunk > 65536) {
       chunk = round_up(chunk / 2, kStrideAlign);
@@ -251,14 +265,16 @@ xmpError_t XMPAPI xmpIntegersPowmAsync(xmpHandle_t handle, xmpIntegers_t out, co
       return xmpErrorIncreaseScratchLimit;
     }
   }
-  size_t need = fixed + powm_table_words(v, round_up(chunk, kStrideAlign), wbits) * sizeof(uint32_t);
+  size_t need = fixed + table_words(chunk, wbits) * sizeof(uint32_t);
   e = set_necessary_scratch(handle, need);
   if (e != xmpErrorSuccess) return e;
   uint32_t *r2 = (uint32_t *)handle->scratch;
   uint32_t *divws = r2 + r2_words;
   uint32_t *table = divws + div_words;
-  if (launch_montgomery_r2(c.variant, r2, mstride, mod->limbs, mod->nlimbs, mod->stride, mod->count,
-                           div_words ? divws : nullptr, handle->stream) != hipSuccess)
+  if ((wide ? launch_wide_r2(wide, r2, mstride, mod->limbs, mod->nlimbs, mod->stride, mod->count,
+                             div_words ? divws : nullptr, handle->stream)
+            : launch_montgomery_r2(c.variant, r2, mstride, mod->limbs, mod->nlimbs, mod->stride, mod->count,
+                                   div_words ? divws : nullptr, handle->stream)) != hipSuccess)
     return xmpErrorCuda;
   PowmArgs args;
   args.out = make_out(out, obuf, p, 0);
@@ -272,7 +288,9 @@ xmpError_t XMPAPI xmpIntegersPowmAsync(xmpHandle_t handle, xmpIntegers_t out, co
   for (uint32_t start = 0; start < count; start += chunk) {
     args.start = start;
     args.count = std::min(chunk, count - start);
-    if (launch_powm_variant(c.variant, args, handle->stream) != hipSuccess) return xmpErrorCuda;
+    if ((wide ? launch_powm_wide(wide, args, handle->stream) : launch_powm_variant(c.variant, args, handle->stream)) !=
+        hipSuccess)
+      return xmpErrorCuda;
   }
   if (inplace && hipMemcpyAsync(out->limbs, obuf, obytes, hipMemcpyDeviceToDevice, handle->stream) != hipSuccess)
     return xmpErrorCuda;

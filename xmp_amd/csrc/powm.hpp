@@ -56,6 +56,21 @@ hipError_t launch_powm_variant(int variant, const PowmArgs &args, hipStream_t st
 // kMaxRegDivisorLimbs need `ws` of divmod_scratch_words(m_len, count) words.
 hipError_t launch_montgomery_r2(int variant, uint32_t *r2, uint32_t r2_stride, const uint32_t *m, uint32_t m_len,
                                 uint32_t m_stride, uint32_t count, uint32_t *ws, hipStream_t stream);
+// r2 = v - m if v >= m else v (v < 2m, m_len + 1 limbs), one lane per modulus
+hipError_t launch_r2_canonical(uint32_t *r2, uint32_t r2_stride, const uint32_t *v, uint32_t v_stride,
+                               const uint32_t *m, uint32_t m_len, uint32_t m_stride, uint32_t count,
+                               hipStream_t stream);
+
+// Moduli wider than every lane layout: one 256-thread workgroup per instance
+// with C radix-2^24 digits per thread (C = 8, 16, 32; R = 2^(6144 C)).
+// digits per thread for a precision, or 0 above the widest (196,606 bits)
+uint32_t powm_wide_digits_per_thread(uint32_t precision);
+uint32_t powm_wide_rbits(uint32_t c);
+size_t powm_wide_table_words(uint32_t c, uint32_t count, uint32_t window_bits);
+hipError_t launch_powm_wide(uint32_t c, const PowmArgs &args, hipStream_t stream);
+// R^2 mod m for the wide layout (needs `ws` of divmod_scratch_words(m_len, count) words)
+hipError_t launch_wide_r2(uint32_t c, uint32_t *r2, uint32_t r2_stride, const uint32_t *m, uint32_t m_len,
+                          uint32_t m_stride, uint32_t count, uint32_t *ws, hipStream_t stream);
 // variants usable for a modulus precision (in bits): those with R >= 2^(precision+2)
 bool variant_fits(const PowmVariant &v, uint32_t precision);
 // whether the persistent mixed-size kernel runs this variant
