@@ -82,6 +82,17 @@ def cpu_cores():
     return max(1, min(16, n))
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def powm_workload(h, n, bits, seed):
     limbs = bits // 32
     m = inputs.odd_moduli(seed, n, limbs)
@@ -411,7 +422,15 @@ def main():
                    "sample": "GMP %s mpz_powm (the reference test suite's oracle, unit_tests/test.cu:1214) on the "
                              "first %d of the same 2048-bit instances, %d pthreads, %.1f s wall" % (
                                  O.gmp_version(), S, cores, secs)}
-            log("cpu baseline: %.1f powm/s on %d cores" % (S / secs, cores))
+            # mpz_powm_sec, the constant-time routine the reference's perf harness
+            # times (perf_tests/main.cu:205), on a quarter of that sample
+            S2 = S // 4
+            want2, _, secs2 = O.gmp_batch(O.GO_POWM_SEC, S2, 64, b[:S2], e[:S2], m[:S2], threads=cores)
+            cpu["powm_sec_value"] = round(S2 / secs2, 1)
+            cpu["host_cpu"] = "%s, %d cores visible" % (cpu_model(), os.cpu_count() or 0)
+            log("cpu baseline: %.1f powm/s (mpz_powm_sec %.1f) on %d cores" % (S / secs, S2 / secs2, cores))
+            if not np.array_equal(want[:S2], want2):
+                raise SystemExit("mpz_powm and mpz_powm_sec disagree on the CPU sample")
             parity["headline"] = bool(np.array_equal(got[:S], want))
             if not args.quick:
                 ok = True
