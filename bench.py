@@ -294,6 +294,9 @@ def main():
     ctx.close()
 
     extra = {}
+    # GMP counterparts of the secondary figures, timed in the CPU leg:
+    # (extra key, GMP op, count, out limbs, a, b, c)
+    cpu_work = []
     if not args.quick:
         # ---- the same 65,536 x 2048-bit work as one persistent graph launch vs the plain call
         OUT2 = h.integers(BITS, N)
@@ -370,6 +373,8 @@ def main():
                                   "GB_per_s": round(1024 * n2 / mul_ms / 1e6, 1),
                                   "TOP_per_s": round(2 * 4096 * n2 / mul_ms / 1e9, 3)}
         log("add %.3f ms  sub %.3f ms  xor %.3f ms  mul %.3f ms" % (add_ms, sub_ms, xor_ms, mul_ms))
+        cpu_work += [("add_2048", "GO_ADD", n2, 64, a2, b2, None), ("sub_2048", "GO_SUB", n2, 64, a2, b2, None),
+                     ("mul_2048x2048", "GO_MUL", n2, 128, a2, b2, None)]
         del A2, B2, S2, P2
 
         # ---- div/mod: 65,536 x (4096-bit / 2048-bit, divisor top bit set)
@@ -387,6 +392,8 @@ def main():
         extra["div_4096_by_2048"] = {"count": N, "ms": round(div_ms, 4), "div_per_s": round(N / div_ms * 1e3, 1)}
         extra["mod_4096_by_2048"] = {"count": N, "ms": round(mod_ms, 4), "mod_per_s": round(N / mod_ms * 1e3, 1)}
         log("div %.3f ms  mod %.3f ms" % (div_ms, mod_ms))
+        cpu_work += [("div_4096_by_2048", "GO_DIV", N, 128, a3, d3, None),
+                     ("mod_4096_by_2048", "GO_MOD", N, 64, a3, d3, None)]
 
         # ---- 4,096 x 1024-bit powm (the reference's CPU-runnable case)
         m1, b1, e1, M1, B1, E1 = powm_workload(h, 4096, 1024, 1024)
@@ -397,6 +404,7 @@ def main():
         p1_ms = time_async(p1, stream, 5)
         extra["powm_1024_x4096"] = {"ms": round(p1_ms, 4), "powm_per_s": round(4096 / p1_ms * 1e3, 1)}
         log("powm 1024 x4096: %.3f ms" % p1_ms)
+        cpu_work.append(("powm_1024_x4096", "GO_POWM", 4096, 32, b1, e1, m1))
 
         # ---- mixed-size batch: 262,144 instances, one persistent launch vs plain per-class launches
         mixed, mixed_samples = run_mixed(h, stream, 262144, 777)
@@ -433,11 +441,21 @@ def main():
                 raise SystemExit("mpz_powm and mpz_powm_sec disagree on the CPU sample")
             parity["headline"] = bool(np.array_equal(got[:S], want))
             if not args.quick:
-                ok = True
+                for key, opname, n, ol, xa, xb, xc in cpu_work:
+                    _, _, t = O.gmp_batch(getattr(O, opname), n, ol, xa, xb, xc, threads=cores)
+                    extra[key]["cpu_gmp_per_s"] = round(n / t, 1)
+                    log("cpu %s: %.1f /s" % (key, n / t))
+                # every class's sampled results; the samples (one instance per thread)
+                # also give GMP's per-core rate per class, from which the whole mixed
+                # draw's time on the same cores is estimated
+                ok, est_s = True, 0.0
                 for bits, bb, ee, mm, gg in mixed_samples:
-                    w, _, _ = O.gmp_batch(O.GO_POWM, bb.shape[0], bits // 32, bb, ee, mm, threads=cores)
+                    k = bb.shape[0]
+                    w, _, t = O.gmp_batch(O.GO_POWM, k, bits // 32, bb, ee, mm, threads=min(cores, k))
                     ok &= bool(np.array_equal(gg, w))
+                    est_s += mixed["class_counts"][str(bits)] * t / cores
                 parity["mixed_batch_sample"] = ok
+                extra["mixed_batch"]["cpu_gmp_estimated_ms"] = round(est_s * 1e3, 1)
                 keys, kk, ki, bb, ee, gg = serve_sample
                 parity["serving_sample"] = bool(np.array_equal(
                     gg, O.serving_expected(keys, kk, ki, bb, ee, threads=cores)))
