@@ -69,7 +69,7 @@ def build(verbose=False):
         raise RuntimeError("HIP build failed: " + ", ".join(s for s, _ in failed))
     objs = [os.path.join(OBJ, s + ".o") for s in sources]
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread", "-lrocprofiler-sdk-roctx"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

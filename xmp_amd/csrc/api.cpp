@@ -188,6 +188,7 @@ xmpError_t XMPAPI xmpExecutionPolicyDestroy(xmpHandle_t handle, xmpExecutionPoli
 // reference xmp.cu:238-280: indices are copied into device memory owned by the policy
 xmpError_t XMPAPI xmpExecutionPolicySetIndicesAsync(xmpHandle_t handle, xmpExecutionPolicy_t policy,
                                                     uint32_t which_integer, uint32_t *indices, uint32_t count) {
+  XMP_TRACE("xmpExecutionPolicySetIndicesAsync");
   XMP_SET_DEVICE(handle);
   if (which_integer >= kMaxIndexArrays) return xmpErrorInvalidParameter;
   uint32_t **pind = &policy->indices[which_integer];
@@ -343,10 +344,12 @@ static xmpError_t import_internal(xmpHandle_t handle, xmpIntegers_t out, uint32_
 
 xmpError_t XMPAPI xmpIntegersImport(xmpHandle_t handle, xmpIntegers_t out, uint32_t words, int32_t order, size_t size,
                                     int32_t endian, uint32_t nails, void *in, uint32_t count) {
+  XMP_TRACE("xmpIntegersImport");
   return import_internal(handle, out, words, order, size, endian, nails, in, count, false);
 }
 xmpError_t XMPAPI xmpIntegersImportAsync(xmpHandle_t handle, xmpIntegers_t out, uint32_t words, int32_t order,
                                          size_t size, int32_t endian, uint32_t nails, void *in, uint32_t count) {
+  XMP_TRACE("xmpIntegersImportAsync");
   return import_internal(handle, out, words, order, size, endian, nails, in, count, true);
 }
 
@@ -396,16 +399,19 @@ static xmpError_t export_internal(xmpHandle_t handle, void *out, uint32_t *words
 
 xmpError_t XMPAPI xmpIntegersExport(xmpHandle_t handle, void *out, uint32_t *words, int32_t order, size_t size,
                                     int32_t endian, uint32_t nails, xmpIntegers_t in, uint32_t count) {
+  XMP_TRACE("xmpIntegersExport");
   return export_internal(handle, out, words, order, size, endian, nails, in, count, false);
 }
 xmpError_t XMPAPI xmpIntegersExportAsync(xmpHandle_t handle, void *out, uint32_t *words, int32_t order, size_t size,
                                          int32_t endian, uint32_t nails, xmpIntegers_t in, uint32_t count) {
+  XMP_TRACE("xmpIntegersExportAsync");
   return export_internal(handle, out, words, order, size, endian, nails, in, count, true);
 }
 
 // reference xmp.cu:734-813; the reorder runs on the handle's stream (the
 // reference launches it on the default stream, xmp.cu:806).
 xmpError_t XMPAPI xmpIntegersSetAsync(xmpHandle_t handle, xmpIntegers_t out, xmpIntegers_t in, uint32_t count) {
+  XMP_TRACE("xmpIntegersSetAsync");
   if (in->device != handle->device && out->device != handle->device) return xmpErrorInvalidDevice;
   XMP_SET_DEVICE(handle);
   if (in->precision != out->precision) return xmpErrorInvalidPrecision;

@@ -78,6 +78,7 @@ extern "C" {
 xmpError_t XMPAPI xmpBatchCreate(xmpHandle_t handle, xmpBatch_t *batch, uint32_t classes, xmpIntegers_t *out,
                                  xmpIntegers_t *base, xmpIntegers_t *exp, xmpIntegers_t *mod,
                                  const uint32_t *counts) {
+  XMP_TRACE("xmpBatchCreate");
   XMP_CHECK_NE(batch, nullptr);
   if (classes == 0 || !out || !base || !exp || !mod || !counts) return xmpErrorInvalidParameter;
   const int dev = handle->device;
@@ -192,6 +193,7 @@ xmpError_t XMPAPI xmpBatchDestroy(xmpHandle_t handle, xmpBatch_t batch) {
 }
 
 xmpError_t XMPAPI xmpBatchPrepare(xmpHandle_t handle, xmpBatch_t batch) {
+  XMP_TRACE("xmpBatchPrepare");
   XMP_CHECK_NE(batch, nullptr);
   XMP_SET_DEVICE(handle);
   for (uint32_t c = 0; c < batch->classes; c++) {
@@ -211,6 +213,7 @@ xmpError_t XMPAPI xmpBatchPrepare(xmpHandle_t handle, xmpBatch_t batch) {
 }
 
 xmpError_t XMPAPI xmpBatchLaunchAsync(xmpHandle_t handle, xmpBatch_t batch) {
+  XMP_TRACE("xmpBatchLaunchAsync");
   XMP_CHECK_NE(batch, nullptr);
   XMP_SET_DEVICE(handle);
   BatchArgs ba{batch->d_classes, batch->d_variant, batch->d_tile_class, batch->d_tile_index, batch->ntiles,
@@ -237,6 +240,7 @@ xmpError_t XMPAPI xmpBatchLaunchAsync(xmpHandle_t handle, xmpBatch_t batch) {
 }
 
 xmpError_t XMPAPI xmpBatchLaunchPlainAsync(xmpHandle_t handle, xmpBatch_t batch) {
+  XMP_TRACE("xmpBatchLaunchPlainAsync");
   XMP_CHECK_NE(batch, nullptr);
   xmpExecutionPolicy_t saved = handle->policy;
   handle->policy = const_cast<xmpExecutionPolicy_t>(&kDefaultPolicy);

@@ -63,6 +63,7 @@ extern "C" {
 
 xmpError_t XMPAPI xmpMontgomeryCreate(xmpHandle_t handle, xmpMontgomery_t *ctx, xmpIntegers_t mod,
                                       uint32_t exp_precision, uint32_t count_hint) {
+  XMP_TRACE("xmpMontgomeryCreate");
   XMP_CHECK_NE(ctx, nullptr);
   XMP_CHECK_NE(mod, nullptr);
   if (mod->device != handle->device) return xmpErrorInvalidDevice;
@@ -132,6 +133,7 @@ xmpError_t XMPAPI xmpMontgomeryGetR2(xmpHandle_t handle, xmpMontgomery_t ctx, xm
 
 xmpError_t XMPAPI xmpIntegersPowmMontgomeryAsync(xmpHandle_t handle, xmpIntegers_t out, const xmpIntegers_t base,
                                                  const xmpIntegers_t exp, xmpMontgomery_t ctx, uint32_t count) {
+  XMP_TRACE("xmpIntegersPowmMontgomeryAsync");
   XMP_CHECK_NE(ctx, nullptr);
   xmpExecutionPolicy_t p = handle->policy;
   const int dev = handle->device;

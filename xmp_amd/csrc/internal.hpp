@@ -12,6 +12,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstddef>
 #include <cstdint>
@@ -20,6 +21,17 @@
 #include "../../include/xmp.h"
 
 namespace xmp {
+
+// A roctx range over one API call, so that `rocprofv3 --marker-trace` shows
+// each call of the library by name beside the kernels it launched.  Without
+// a profiler attached a range costs one call into the roctx stub.
+struct TraceRange {
+  explicit TraceRange(const char *name) { roctxRangePushA(name); }
+  ~TraceRange() { roctxRangePop(); }
+  TraceRange(const TraceRange &) = delete;
+  TraceRange &operator=(const TraceRange &) = delete;
+};
+#define XMP_TRACE(name) ::xmp::TraceRange xmp_trace_range_(name)
 
 constexpr uint32_t kMaxIndexArrays = 4;   // reference xmp_internal.h:59
 constexpr uint32_t kStrideAlign = 64;     // instances per wave64

@@ -85,6 +85,7 @@ extern "C" {
 // c may be host or device memory; host results land when the stream completes.
 xmpError_t XMPAPI xmpIntegersCmpAsync(xmpHandle_t handle, int32_t *c, const xmpIntegers_t a, const xmpIntegers_t b,
                                       uint32_t count) {
+  XMP_TRACE("xmpIntegersCmpAsync");
   XMP_CHECK_NE(c, nullptr);
   if (a->device != handle->device || b->device != handle->device) return xmpErrorInvalidDevice;
   if (count == 0) return xmpErrorSuccess;
@@ -113,6 +114,7 @@ xmpError_t XMPAPI xmpIntegersCmp(xmpHandle_t handle, int32_t *c, const xmpIntege
 // shift may be host or device memory.
 xmpError_t XMPAPI xmpIntegersShfAsync(xmpHandle_t handle, xmpIntegers_t c, const xmpIntegers_t a,
                                       const int32_t *shift, const uint32_t shift_count, uint32_t count) {
+  XMP_TRACE("xmpIntegersShfAsync");
   XMP_CHECK_NE(shift, nullptr);
   xmpExecutionPolicy_t p = handle->policy;
   if (c->device != handle->device || a->device != handle->device) return xmpErrorInvalidDevice;
@@ -156,6 +158,7 @@ xmpError_t XMPAPI xmpIntegersShf(xmpHandle_t handle, xmpIntegers_t c, const xmpI
 
 xmpError_t XMPAPI xmpIntegersIorAsync(xmpHandle_t handle, xmpIntegers_t c, const xmpIntegers_t a,
                                       const xmpIntegers_t b, uint32_t count) {
+  XMP_TRACE("xmpIntegersIorAsync");
   return bitwise_async(kBitIor, handle, c, a, b, count);
 }
 xmpError_t XMPAPI xmpIntegersIor(xmpHandle_t handle, xmpIntegers_t c, const xmpIntegers_t a, const xmpIntegers_t b,
@@ -164,6 +167,7 @@ xmpError_t XMPAPI xmpIntegersIor(xmpHandle_t handle, xmpIntegers_t c, const xmpI
 }
 xmpError_t XMPAPI xmpIntegersAndAsync(xmpHandle_t handle, xmpIntegers_t c, const xmpIntegers_t a,
                                       const xmpIntegers_t b, uint32_t count) {
+  XMP_TRACE("xmpIntegersAndAsync");
   return bitwise_async(kBitAnd, handle, c, a, b, count);
 }
 xmpError_t XMPAPI xmpIntegersAnd(xmpHandle_t handle, xmpIntegers_t c, const xmpIntegers_t a, const xmpIntegers_t b,
@@ -172,6 +176,7 @@ xmpError_t XMPAPI xmpIntegersAnd(xmpHandle_t handle, xmpIntegers_t c, const xmpI
 }
 xmpError_t XMPAPI xmpIntegersXorAsync(xmpHandle_t handle, xmpIntegers_t c, const xmpIntegers_t a,
                                       const xmpIntegers_t b, uint32_t count) {
+  XMP_TRACE("xmpIntegersXorAsync");
   return bitwise_async(kBitXor, handle, c, a, b, count);
 }
 xmpError_t XMPAPI xmpIntegersXor(xmpHandle_t handle, xmpIntegers_t c, const xmpIntegers_t a, const xmpIntegers_t b,
@@ -179,6 +184,7 @@ xmpError_t XMPAPI xmpIntegersXor(xmpHandle_t handle, xmpIntegers_t c, const xmpI
   return sync(handle, xmpIntegersXorAsync(handle, c, a, b, count));
 }
 xmpError_t XMPAPI xmpIntegersNotAsync(xmpHandle_t handle, xmpIntegers_t c, const xmpIntegers_t a, uint32_t count) {
+  XMP_TRACE("xmpIntegersNotAsync");
   return bitwise_async(kBitNot, handle, c, a, nullptr, count);
 }
 xmpError_t XMPAPI xmpIntegersNot(xmpHandle_t handle, xmpIntegers_t c, const xmpIntegers_t a, uint32_t count) {
@@ -188,6 +194,7 @@ xmpError_t XMPAPI xmpIntegersNot(xmpHandle_t handle, xmpIntegers_t c, const xmpI
 // c[i] = number of one bits of instance i (reference logic.cu:541-605).
 // c may be host or device memory.
 xmpError_t XMPAPI xmpIntegersPopcAsync(xmpHandle_t handle, uint32_t *c, const xmpIntegers_t a, uint32_t count) {
+  XMP_TRACE("xmpIntegersPopcAsync");
   XMP_CHECK_NE(c, nullptr);
   if (a->device != handle->device) return xmpErrorInvalidDevice;
   if (count == 0) return xmpErrorSuccess;

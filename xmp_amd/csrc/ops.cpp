@@ -86,6 +86,7 @@ extern "C" {
 
 xmpError_t XMPAPI xmpIntegersAddAsync(xmpHandle_t handle, xmpIntegers_t s, const xmpIntegers_t a,
                                       const xmpIntegers_t b, uint32_t count) {
+  XMP_TRACE("xmpIntegersAddAsync");
   return binary_async(Bin::Add, handle, s, a, b, count);
 }
 xmpError_t XMPAPI xmpIntegersAdd(xmpHandle_t handle, xmpIntegers_t s, const xmpIntegers_t a, const xmpIntegers_t b,
@@ -94,6 +95,7 @@ xmpError_t XMPAPI xmpIntegersAdd(xmpHandle_t handle, xmpIntegers_t s, const xmpI
 }
 xmpError_t XMPAPI xmpIntegersSubAsync(xmpHandle_t handle, xmpIntegers_t d, const xmpIntegers_t a,
                                       const xmpIntegers_t b, uint32_t count) {
+  XMP_TRACE("xmpIntegersSubAsync");
   return binary_async(Bin::Sub, handle, d, a, b, count);
 }
 xmpError_t XMPAPI xmpIntegersSub(xmpHandle_t handle, xmpIntegers_t d, const xmpIntegers_t a, const xmpIntegers_t b,
@@ -101,6 +103,7 @@ xmpError_t XMPAPI xmpIntegersSub(xmpHandle_t handle, xmpIntegers_t d, const xmpI
   return sync(handle, xmpIntegersSubAsync(handle, d, a, b, count));
 }
 xmpError_t XMPAPI xmpIntegersSqrAsync(xmpHandle_t handle, xmpIntegers_t p, const xmpIntegers_t a, uint32_t count) {
+  XMP_TRACE("xmpIntegersSqrAsync");
   xmpExecutionPolicy_t pol = handle->policy;
   if (p->device != handle->device || a->device != handle->device) return xmpErrorInvalidDevice;
   if (count == 0) return xmpErrorSuccess;
@@ -124,6 +127,7 @@ xmpError_t XMPAPI xmpIntegersSqrAsync(xmpHandle_t handle, xmpIntegers_t p, const
 }
 xmpError_t XMPAPI xmpIntegersMulAsync(xmpHandle_t handle, xmpIntegers_t p, const xmpIntegers_t a,
                                       const xmpIntegers_t b, uint32_t count) {
+  XMP_TRACE("xmpIntegersMulAsync");
   xmpExecutionPolicy_t pol = handle->policy;
   // reference mul.cu:147-149: a == b without input indices is a square
   if (a == b && !pol->indices[1] && !pol->indices[2]) return xmpIntegersSqrAsync(handle, p, a, count);
@@ -178,6 +182,7 @@ static xmpError_t divmod_async(xmpHandle_t h, xmpIntegers_t q, xmpIntegers_t m, 
 
 xmpError_t XMPAPI xmpIntegersDivAsync(xmpHandle_t handle, xmpIntegers_t q, const xmpIntegers_t a,
                                       const xmpIntegers_t b, uint32_t count) {
+  XMP_TRACE("xmpIntegersDivAsync");
   return divmod_async(handle, q, nullptr, a, b, count, false);
 }
 xmpError_t XMPAPI xmpIntegersDiv(xmpHandle_t handle, xmpIntegers_t q, const xmpIntegers_t a, const xmpIntegers_t b,
@@ -186,6 +191,7 @@ xmpError_t XMPAPI xmpIntegersDiv(xmpHandle_t handle, xmpIntegers_t q, const xmpI
 }
 xmpError_t XMPAPI xmpIntegersModAsync(xmpHandle_t handle, xmpIntegers_t m, const xmpIntegers_t a,
                                       const xmpIntegers_t b, uint32_t count) {
+  XMP_TRACE("xmpIntegersModAsync");
   return divmod_async(handle, nullptr, m, a, b, count, false);
 }
 xmpError_t XMPAPI xmpIntegersMod(xmpHandle_t handle, xmpIntegers_t m, const xmpIntegers_t a, const xmpIntegers_t b,
@@ -194,6 +200,7 @@ xmpError_t XMPAPI xmpIntegersMod(xmpHandle_t handle, xmpIntegers_t m, const xmpI
 }
 xmpError_t XMPAPI xmpIntegersDivModAsync(xmpHandle_t handle, xmpIntegers_t q, xmpIntegers_t m,
                                          const xmpIntegers_t a, const xmpIntegers_t b, uint32_t count) {
+  XMP_TRACE("xmpIntegersDivModAsync");
   if (q == nullptr || m == nullptr) return xmpErrorInvalidParameter;
   return divmod_async(handle, q, m, a, b, count, true);
 }
@@ -205,6 +212,7 @@ xmpError_t XMPAPI xmpIntegersDivMod(xmpHandle_t handle, xmpIntegers_t q, xmpInte
 // ---- powm --------------------------------------------------------------
 xmpError_t XMPAPI xmpIntegersPowmAsync(xmpHandle_t handle, xmpIntegers_t out, const xmpIntegers_t a,
                                        const xmpIntegers_t exp, const xmpIntegers_t mod, uint32_t count) {
+  XMP_TRACE("xmpIntegersPowmAsync");
   xmpExecutionPolicy_t p = handle->policy;
   const int dev = handle->device;
   if (out->device != dev || a->device != dev || exp->device != dev || mod->device != dev)

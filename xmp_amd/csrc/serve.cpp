@@ -156,6 +156,7 @@ const _xmpServer_t::Layout &pick_layout(const std::vector<_xmpServer_t::Layout> 
 // second stream, each with its own input copy, kernels and result copy), then
 // the completion event once both have finished.
 hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
+  XMP_TRACE("xmpServer batch");
   const uint32_t n32 = s->n32, h32 = s->h32;
   const uint32_t na = sl.na, nb = sl.nb, cap = sl.cap;
   hipStream_t st = sl.stream;
@@ -257,6 +258,7 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
 
 void worker_loop(xmpServer_t s) {
   (void)hipSetDevice(s->h->device);
+  roctxNameOsThread("xmpServer scheduler");
   const uint32_t n32 = s->n32;
   while (true) {
     bool progressed = false;
