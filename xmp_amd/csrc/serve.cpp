@@ -303,6 +303,8 @@ void worker_loop(xmpServer_t s) {
         if (take == 0) break;
         // kind-0 requests first, then kind 1 (slot layout); order within a kind is arrival order
         std::vector<Request> a, b;
+        a.reserve(take);
+        b.reserve(take);
         for (uint32_t i = 0; i < take; i++) {
           Request r = std::move(s->pending.front());
           s->pending.pop_front();
