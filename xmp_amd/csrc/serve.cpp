@@ -99,7 +99,10 @@ struct _xmpServer_t {
     uint32_t *h_in = nullptr;     // [A base | A exp | B c] compact, cap*n32 each
     uint32_t *h_idx = nullptr;    // index arrays
     uint32_t *h_out = nullptr;    // [A out | B out] compact
-    uint32_t *d_in = nullptr, *d_idx = nullptr, *d_out = nullptr;
+    // device addresses of the three pinned buffers: the slot's kernels read the
+    // requests and write the results there directly, with no copy-engine transfer
+    uint32_t *g_in = nullptr, *g_idx = nullptr, *g_out = nullptr;
+    uint32_t *d_idx = nullptr;
     DevArray a_base, a_exp, a_out, b_c, b_cm, b_pow, b_prod, b_sum, b_out;
     uint32_t *table_a = nullptr, *table_b = nullptr;
     std::vector<Request> reqs;
@@ -137,6 +140,11 @@ Operand op(xmpIntegers_t x, const uint32_t *idx = nullptr, uint32_t idx_count = 
 }
 OutOperand outop(const DevArray &a) { return OutOperand{a.data, a.limbs, a.stride, nullptr}; }
 
+// The slots' pinned buffers are mapped and coherent (not cached on the
+// device): the host rewrites them between batches, and the kernels read the
+// requests from them and write the results into them directly.
+constexpr unsigned kPinned = hipHostMallocMapped | hipHostMallocCoherent;
+
 // the layout prepared for the batch size nearest (log scale) to `count`
 const _xmpServer_t::Layout &pick_layout(const std::vector<_xmpServer_t::Layout> &set, uint32_t count) {
   size_t best = 0;
@@ -153,8 +161,9 @@ const _xmpServer_t::Layout &pick_layout(const std::vector<_xmpServer_t::Layout> 
 
 // Launch one batch on its slot's streams: the index staging, then the two
 // request kinds concurrently (kind 0 on the slot's stream, kind 1 on its
-// second stream, each with its own input copy, kernels and result copy), then
-// the completion event once both have finished.
+// second stream, each importing its requests from the pinned input buffer and
+// exporting its results into the pinned output buffer), then the completion
+// event once both have finished.
 hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
   XMP_TRACE("xmpServer batch");
   const uint32_t n32 = s->n32, h32 = s->h32;
@@ -177,21 +186,18 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
   uint32_t *d_keyA = sl.d_idx, *d_keyPQ = d_keyA + cap, *d_cidx = d_keyPQ + 2 * cap, *d_lo = d_cidx + 2 * cap,
            *d_hi = d_lo + cap, *d_keyB = d_hi + cap;
   hipError_t e;
-  if ((e = hipMemcpyAsync(sl.d_idx, sl.h_idx, sizeof(uint32_t) * kIdxWords * cap, hipMemcpyHostToDevice, st)) != hipSuccess)
+  // the index arrays are re-read by the kernels, so they are staged in device
+  // memory (a one-limb import is a word copy)
+  if ((e = launch_import(sl.d_idx, kIdxWords * cap, 1, (const uint8_t *)sl.g_idx, kIdxWords * cap, 1, -1, -1, 0, 4,
+                         nullptr, 0, st)) != hipSuccess)
     return e;
   hipStream_t sb = sl.stream_b;
   if ((e = hipEventRecord(sl.fork, st)) || (e = hipStreamWaitEvent(sb, sl.fork, 0))) return e;
   const size_t words = (size_t)cap * n32;
-  // only the rows this batch uses: bases and exponents of the na powm requests, c of the nb CRT requests
-  if (na && ((e = hipMemcpyAsync(sl.d_in, sl.h_in, sizeof(uint32_t) * na * n32, hipMemcpyHostToDevice, st)) ||
-             (e = hipMemcpyAsync(sl.d_in + words, sl.h_in + words, sizeof(uint32_t) * na * n32,
-                                 hipMemcpyHostToDevice, st))))
-    return e;
-  if (nb && (e = hipMemcpyAsync(sl.d_in + 2 * words, sl.h_in + 2 * words, sizeof(uint32_t) * nb * n32,
-                                hipMemcpyHostToDevice, sb)))
-    return e;
-  const uint8_t *in_base = (const uint8_t *)sl.d_in, *in_exp = (const uint8_t *)(sl.d_in + words),
-                *in_c = (const uint8_t *)(sl.d_in + 2 * words);
+  // the imports read the rows this batch uses straight from the pinned buffer:
+  // bases and exponents of the na powm requests, c of the nb CRT requests
+  const uint8_t *in_base = (const uint8_t *)sl.g_in, *in_exp = (const uint8_t *)(sl.g_in + words),
+                *in_c = (const uint8_t *)(sl.g_in + 2 * words);
   if (na) {
     if ((e = launch_import(sl.a_base.data, sl.a_base.stride, n32, in_base, na, n32, -1, -1, 0, 4, nullptr, 0, st)) ||
         (e = launch_import(sl.a_exp.data, sl.a_exp.stride, n32, in_exp, na, n32, -1, -1, 0, 4, nullptr, 0, st)))
@@ -209,7 +215,7 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
     a.count = na;
     a.window_bits = ly.window;
     if ((e = launch_powm_variant(ly.variant, a, st)) != hipSuccess) return e;
-    if ((e = launch_export((uint8_t *)sl.d_out, sl.a_out.data, sl.a_out.stride, n32, na, n32, -1, -1, 0, 4, nullptr,
+    if ((e = launch_export((uint8_t *)sl.g_out, sl.a_out.data, sl.a_out.stride, n32, na, n32, -1, -1, 0, 4, nullptr,
                            0, st)) != hipSuccess)
       return e;
   }
@@ -242,15 +248,10 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
     if ((e = launch_divmod_ws(none, outop(sl.b_out), op(sl.b_sum), op(s->N, d_keyB, nb), nb, nullptr, sb)) !=
         hipSuccess)
       return e;
-    if ((e = launch_export((uint8_t *)(sl.d_out + words), sl.b_out.data, sl.b_out.stride, n32, nb, n32, -1, -1, 0, 4,
+    if ((e = launch_export((uint8_t *)(sl.g_out + words), sl.b_out.data, sl.b_out.stride, n32, nb, n32, -1, -1, 0, 4,
                            nullptr, 0, sb)) != hipSuccess)
       return e;
   }
-  if (na && (e = hipMemcpyAsync(sl.h_out, sl.d_out, sizeof(uint32_t) * na * n32, hipMemcpyDeviceToHost, st)))
-    return e;
-  if (nb && (e = hipMemcpyAsync(sl.h_out + words, sl.d_out + words, sizeof(uint32_t) * nb * n32,
-                                hipMemcpyDeviceToHost, sb)))
-    return e;
   if ((e = hipEventRecord(sl.join, sb)) || (e = hipStreamWaitEvent(st, sl.join, 0))) return e;
   (void)h32;
   return hipEventRecord(sl.done, st);
@@ -350,9 +351,7 @@ void free_server(xmpServer_t s) {
       if (a->data) h->df(a->data);
     if (sl.table_a) h->df(sl.table_a);
     if (sl.table_b) h->df(sl.table_b);
-    if (sl.d_in) h->df(sl.d_in);
     if (sl.d_idx) h->df(sl.d_idx);
-    if (sl.d_out) h->df(sl.d_out);
     if (sl.h_in) (void)hipHostFree(sl.h_in);
     if (sl.h_idx) (void)hipHostFree(sl.h_idx);
     if (sl.h_out) (void)hipHostFree(sl.h_out);
@@ -460,16 +459,17 @@ xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const
         hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&sl.fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&sl.join, hipEventDisableTiming) != hipSuccess ||
-        hipHostMalloc((void **)&sl.h_in, sizeof(uint32_t) * 3 * words, 0) != hipSuccess ||
-        hipHostMalloc((void **)&sl.h_idx, sizeof(uint32_t) * kIdxWords * cap, 0) != hipSuccess ||
-        hipHostMalloc((void **)&sl.h_out, sizeof(uint32_t) * 2 * words, 0) != hipSuccess)
+        hipHostMalloc((void **)&sl.h_in, sizeof(uint32_t) * 3 * words, kPinned) != hipSuccess ||
+        hipHostMalloc((void **)&sl.h_idx, sizeof(uint32_t) * kIdxWords * cap, kPinned) != hipSuccess ||
+        hipHostMalloc((void **)&sl.h_out, sizeof(uint32_t) * 2 * words, kPinned) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&sl.g_in, sl.h_in, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&sl.g_idx, sl.h_idx, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&sl.g_out, sl.h_out, 0) != hipSuccess)
       return fail(xmpErrorCuda);
-    sl.d_in = (uint32_t *)handle->da(sizeof(uint32_t) * 3 * words);
     sl.d_idx = (uint32_t *)handle->da(sizeof(uint32_t) * kIdxWords * cap);
-    sl.d_out = (uint32_t *)handle->da(sizeof(uint32_t) * 2 * words);
     sl.table_a = (uint32_t *)handle->da(sizeof(uint32_t) * tab_a);
     sl.table_b = (uint32_t *)handle->da(sizeof(uint32_t) * tab_b);
-    if (!sl.d_in || !sl.d_idx || !sl.d_out || !sl.table_a || !sl.table_b) return fail(xmpErrorInvalidCudaMalloc);
+    if (!sl.d_idx || !sl.table_a || !sl.table_b) return fail(xmpErrorInvalidCudaMalloc);
     if (alloc_array(handle, sl.a_base, s->n32, cap) || alloc_array(handle, sl.a_exp, s->n32, cap) ||
         alloc_array(handle, sl.a_out, s->n32, cap) || alloc_array(handle, sl.b_c, s->n32, cap) ||
         alloc_array(handle, sl.b_cm, s->h32, 2 * cap) || alloc_array(handle, sl.b_pow, s->h32, 2 * cap) ||
