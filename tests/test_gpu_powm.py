@@ -28,7 +28,9 @@ def _powm_case(handle, seed, n, bits, exp_bits=None, reduce_base=True, count_mod
     return Out.get(), b, e, m
 
 
-@pytest.mark.parametrize("bits", [64, 96, 128, 160, 256, 288, 320, 384, 512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192])
+# the reference's instantiation sizes, its odd ones included (test.cu:2393-2910)
+@pytest.mark.parametrize("bits", [64, 96, 128, 160, 256, 288, 320, 384, 416, 512, 544, 768, 1024, 1056, 1536, 2048,
+                                  2080, 3072, 4096, 6144, 8192])
 def test_powm_sizes(handle, bits):
     n = 128 if bits <= 2048 else 64
     got, b, e, m = _powm_case(handle, 100 + bits, n, bits)
@@ -117,6 +119,19 @@ def test_powm_forced_layouts(handle, alg, bits):
     finally:
         handle.set_algorithm(xmp_amd.ALG_DEFAULT)
     assert_rows_equal(got, gmp(O.GO_POWM, n, bits // 32, b, e, m), "forced layout")
+
+
+@pytest.mark.parametrize("bits", [512, 1024, 1056, 2080, 6144])
+def test_powm_digitmp_policy(handle, bits):
+    """The reference's DigitMP instantiations (test.cu:2890-2910): the policy
+    takes every layout the modulus fits, and the results are GMP's."""
+    n = 64 if bits <= 2080 else 16
+    handle.set_algorithm(xmp_amd.ALG_DIGITMP)
+    try:
+        got, b, e, m = _powm_case(handle, 500 + bits, n, bits)
+    finally:
+        handle.set_algorithm(xmp_amd.ALG_DEFAULT)
+    assert_rows_equal(got, gmp(O.GO_POWM, n, bits // 32, b, e, m), "DigitMP policy %d" % bits)
 
 
 def test_powm_unreduced_base_and_short_exponent(handle):
