@@ -158,8 +158,9 @@ def run_serving(h, nkeys, requests, seed):
     nmat = keys["n"][key_idx]
     bases[:, -1] = (bases[:, -1].astype(np.uint64) % nmat[:, -1].astype(np.uint64)).astype(np.uint32)
     exps = inputs.uniform(seed + 2, requests, 64)
-    # 4 slots of 8,192 requests: the same saturation as 4 x 16,384 in a sweep of slot
-    # shapes, with half the queueing delay near it (p99 31 vs 67 ms at 600 K req/s)
+    # 4 slots of 8,192 requests (8 hardware queues): in a sweep of slot shapes the best
+    # saturation of the 4-slot shapes (948 K req/s) and p50/p99 16.8/32.1 ms at 600 K
+    # req/s; 8 slots reached 1.03 M req/s but need 16 hardware queues
     srv = Server(h, keys, 2048, streams=4, max_batch=8192)
     out = {"keys": nkeys, "requests": requests, "mix": "50% 2048-bit powm, 50% CRT pairs of 1024-bit moduli",
            "streams": 4, "max_batch": 8192}
