@@ -31,6 +31,7 @@ SOURCES = [
     "k_div.hip",
     "k_logic.hip",
     "k_powm.hip",
+    "k_powm_latency.hip",
     "k_powm_wide.hip",
     "k_probe.hip",
 ]
@@ -45,7 +46,8 @@ COMMON = [
 # scheduled by LLVM's ILP-oriented iterative GCN scheduler: measured A/B on one
 # MI355X, the 65,536 x 2048-bit powm launch took 48.5-48.7 ms against
 # 49.4-49.7 ms with the default occupancy-oriented scheduler (the kernel runs
-# at its register-limited occupancy of two waves per SIMD either way).
+# at its register-limited occupancy of two waves per SIMD either way).  The
+# few-digit latency layouts (k_powm_latency.hip) keep the default scheduler.
 DEVICE_OPTS = {
     "k_powm.hip": ["-Xarch_device", "-mllvm=-misched=gcn-iterative-ilp"],
 }

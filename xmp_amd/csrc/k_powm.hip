@@ -401,12 +401,38 @@ __global__ void __launch_bounds__(64, 2) powm_kernel(PowmArgs args) {
   powm_tile<L, K, D>(args, blockIdx.x, bbuf);
 }
 
+#ifdef XMP_POWM_LATENCY_UNIT
+}  // namespace
+
+// The few-digit latency layouts, compiled in a unit of their own
+// (k_powm_latency.hip) with the default instruction scheduler, which issues
+// their short steps 1-3 % faster than the ILP-oriented one used for this file.
+hipError_t launch_powm_latency(int lanes, int digits, const PowmArgs &args, hipStream_t stream) {
+  if (args.count == 0) return hipErrorInvalidValue;
+  const dim3 grid(div_round_up(args.count, 64u / (uint32_t)lanes)), block(64);
+  if (lanes == 4 && digits == 10)
+    hipLaunchKernelGGL((powm_kernel<4, 10, 28>), grid, block, 0, stream, args);
+  else if (lanes == 8 && digits == 5)
+    hipLaunchKernelGGL((powm_kernel<8, 5, 28>), grid, block, 0, stream, args);
+  else if (lanes == 8 && digits == 10)
+    hipLaunchKernelGGL((powm_kernel<8, 10, 28>), grid, block, 0, stream, args);
+  else if (lanes == 16 && digits == 5)
+    hipLaunchKernelGGL((powm_kernel<16, 5, 28>), grid, block, 0, stream, args);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+}  // namespace xmp
+#else
+
 struct VariantEntry {
   PowmVariant v;
-  void (*kernel)(PowmArgs);
+  void (*kernel)(PowmArgs);   // null: launched by launch_powm_latency
 };
 
 #define XMP_VARIANT(L, K, D) {{L, K, D}, powm_kernel<L, K, D>}
+#define XMP_LATENCY_VARIANT(L, K, D) {{L, K, D}, nullptr}
 const VariantEntry kVariants[] = {
     // <=1024-bit: one lane per instance
     XMP_VARIANT(1, 12, 28),  // 336-bit R : up to 320-bit moduli
@@ -431,12 +457,13 @@ const VariantEntry kVariants[] = {
     XMP_VARIANT(32, 20, 26), // 16640 : 16384-bit
     XMP_VARIANT(64, 20, 26), // 33280 : 32768-bit
     // low-latency layouts (fewer digits per lane, so one wave finishes sooner)
-    XMP_VARIANT(4, 10, 28),  // 1120 : 1024-bit on four lanes
-    XMP_VARIANT(8, 5, 28),   // 1120 : 1024-bit on eight lanes
-    XMP_VARIANT(8, 10, 28),  // 2240 : 2048-bit on eight lanes
-    XMP_VARIANT(16, 5, 28),  // 2240 : 2048-bit on sixteen lanes
+    XMP_LATENCY_VARIANT(4, 10, 28),  // 1120 : 1024-bit on four lanes
+    XMP_LATENCY_VARIANT(8, 5, 28),   // 1120 : 1024-bit on eight lanes
+    XMP_LATENCY_VARIANT(8, 10, 28),  // 2240 : 2048-bit on eight lanes
+    XMP_LATENCY_VARIANT(16, 5, 28),  // 2240 : 2048-bit on sixteen lanes
 };
 #undef XMP_VARIANT
+#undef XMP_LATENCY_VARIANT
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 // ---- R^2 mod m for long moduli by Montgomery squarings ------------------
@@ -645,6 +672,7 @@ hipError_t launch_powm_batch(const BatchArgs &ba, uint32_t grid, size_t lds_byte
 hipError_t launch_powm_variant(int variant, const PowmArgs &args, hipStream_t stream) {
   if (variant < 0 || variant >= kNumVariants || args.count == 0) return hipErrorInvalidValue;
   const PowmVariant &v = kVariants[variant].v;
+  if (!kVariants[variant].kernel) return launch_powm_latency(v.lanes, v.digits, args, stream);
   const uint32_t per_block = 64u / (uint32_t)v.lanes;
   hipLaunchKernelGGL(kVariants[variant].kernel, dim3(div_round_up(args.count, per_block)), dim3(64), 0, stream, args);
   return hipGetLastError();
@@ -690,3 +718,4 @@ hipError_t launch_montgomery_r2(int variant, uint32_t *r2, uint32_t r2_stride, c
 }
 
 }  // namespace xmp
+#endif  // XMP_POWM_LATENCY_UNIT
