@@ -123,7 +123,11 @@ double model_cost(const PowmVariant &v, uint32_t precision, uint32_t exp_bits, u
   // a wave alone on its SIMD issues the 64-bit multiply at half rate
   const double rounds = std::ceil(waves / slots);
   const double fill = std::min(1.0, waves / (slots / 2.0));
-  return rounds * per_wave * (fill < 1.0 ? 2.0 : 1.0);
+  // more than 40 digits per lane overflow the two-wave register budget: the
+  // 2x55 layout spills over a kilobyte per lane and measured 4-8x slower than
+  // 4x28 at the same 3072-bit sizes
+  const double spill = v.digits > 40 ? 4.0 : 1.0;
+  return rounds * per_wave * (fill < 1.0 ? 2.0 : 1.0) * spill;
 }
 
 xmpError_t choose_powm(xmpHandle_t h, uint32_t precision, uint32_t exp_bits, uint32_t count, xmpAlgorithm_t alg,
