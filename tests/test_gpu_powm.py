@@ -134,6 +134,27 @@ def test_powm_digitmp_policy(handle, bits):
     assert_rows_equal(got, gmp(O.GO_POWM, n, bits // 32, b, e, m), "DigitMP policy %d" % bits)
 
 
+def test_cost_model_avoids_spilling_layout(handle):
+    """Without a layout table the cost model picks the layout: for 39,514
+    3072-bit instances it must not pick 2x55, which spills (over a kilobyte
+    per lane) and ran 4x slower than the model's choice."""
+    import os
+    from xmp_amd.ext import Montgomery, tuner_load
+    with open(os.path.join(os.path.dirname(xmp_amd.LIB_PATH), "tune_gfx950.json")) as f:
+        table = f.read()
+    tuner_load('{"device": "gfx950", "entries": []}')
+    try:
+        M = handle.from_array(inputs.odd_moduli(78, 64, 96))
+        ctx = Montgomery(handle, M, 3072, 39514)
+        lanes, digits, radix = ctx.layout()[:3]
+        ctx.close()
+        M.close()
+    finally:
+        tuner_load(table)
+    assert (lanes, digits) != (2, 55)
+    assert lanes * digits * radix >= 3072 + 2
+
+
 def test_powm_unreduced_base_and_short_exponent(handle):
     n = 200
     got, b, e, m = _powm_case(handle, 401, n, 2048, exp_bits=32, reduce_base=False)
