@@ -192,11 +192,11 @@ struct Mont {
     for (int li = 0; li < L; li++) {
       // fresh copies behind an empty asm each iteration, so the compiler
       // cannot hoist the digits' 64-bit extensions out of the loop (that
-      // doubles the registers they take and spills).  Two- and four-lane
-      // groups of up to 38 digits fit their registers without it under the
+      // doubles the registers they take and spills).  Groups of two to eight
+      // lanes with up to 38 digits fit their registers without it under the
       // ILP scheduler (no spill at all for 2x37, whose launch is then
-      // 1.2-1.8 % faster; 4x38 spills less and runs 0.5 % faster); 2x55 keeps it.
-      constexpr bool kCopyBarrier = !((L == 2 || L == 4) && K <= 38);
+      // 1.2-1.8 % faster; 4x38 and 8x38 run 0.5 % faster); 2x55 keeps it.
+      constexpr bool kCopyBarrier = !((L == 2 || L == 4 || L == 8) && K <= 38);
       uint32_t ac[K];
 #pragma unroll
       for (int k = 0; k < K; k++) {
