@@ -114,7 +114,8 @@ def test_add_in_place(handle):
                                                   (1536, 1536, 3072), (1536, 512, 1600), (2048, 2048, 4160),
                                                   (2048, 1024, 2080), (4096, 2048, 4096), (8192, 8192, 16384),
                                                   (8192, 4096, 8192), (12288, 64, 12352), (16384, 16384, 32768),
-                                                  (16384, 16384, 16384), (32, 2048, 2080)])
+                                                  (16384, 16384, 16384), (32, 2048, 2080),
+                                                  (32768, 32768, 65536), (32768, 16384, 32768)])
 def test_mul(handle, bits_a, bits_b, bits_p):
     n = 500
     a, b = _pair(51, n, bits_a // 32, bits_b // 32)

@@ -56,3 +56,79 @@ def test_server_alone_and_in_mix(handle):
         assert_rows_equal(res, want[:100], "one request admitted at a time")
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("bits,nkeys,nreq", [(8192, 4, 16), (16384, 2, 6)])
+def test_server_wide_keys(handle, bits, nkeys, nreq):
+    """Keys above 4096 bits: the CRT chain's final division by n (256 limbs at
+    8192 bits) and, at 16384 bits, also the division by p and q (256 limbs)
+    go past the register division and use the slot's division workspace."""
+    keys = make_keys(nkeys, bits, 500 + bits)
+    rng = np.random.default_rng(bits)
+    kinds = np.array([i % 2 for i in range(nreq)], np.uint32)
+    key_idx = rng.integers(0, nkeys, nreq).astype(np.uint32)
+    bases = inputs.uniform(bits + 1, nreq, bits // 32)
+    nmat = keys["n"][key_idx]
+    bases[:, -1] = (bases[:, -1].astype(np.uint64) % nmat[:, -1].astype(np.uint64)).astype(np.uint32)
+    exps = inputs.uniform(bits + 2, nreq, bits // 32)
+    want = O.serving_expected(keys, kinds, key_idx, bases, exps, threads=THREADS)
+    srv = Server(handle, keys, bits, streams=2, max_batch=8)
+    try:
+        res, lat, wall = srv.run_poisson(1e6, kinds, key_idx, bases, exps, seed=9)
+    finally:
+        srv.close()
+    assert_rows_equal(res, want, "served %d-bit requests" % bits)
+
+
+def test_server_failed_batch_is_reported(handle, monkeypatch):
+    """A batch whose launch fails part-way (forced from the host after its
+    powm requests were enqueued) is not published: the server waits for what
+    was enqueued, reports xmpErrorCuda from Drain/RunPoisson, admits nothing
+    more, and shuts down cleanly."""
+    import xmp_amd
+    keys = make_keys(8, BITS, 78)
+    kinds, key_idx, bases, exps = _requests(keys, 200, 6)
+    monkeypatch.setenv("XMP_SERVER_FAIL_BATCH", "2")
+    srv = Server(handle, keys, BITS, streams=2, max_batch=16)
+    monkeypatch.delenv("XMP_SERVER_FAIL_BATCH")
+    try:
+        with pytest.raises(xmp_amd.XmpError) as err:
+            srv.run_poisson(1e6, kinds, key_idx, bases, exps, seed=5)
+        assert err.value.code == xmp_amd.CUDA_ERROR
+    finally:
+        srv.close()
+    # a server created afterwards (no failure forced) serves normally
+    want = O.serving_expected(keys, kinds[:50], key_idx[:50], bases[:50], exps[:50], threads=THREADS)
+    srv = Server(handle, keys, BITS, streams=2, max_batch=16)
+    try:
+        res, lat, wall = srv.run_poisson(1e6, kinds[:50], key_idx[:50], bases[:50], exps[:50], seed=5)
+    finally:
+        srv.close()
+    assert_rows_equal(res, want, "served after a failed server")
+
+
+def test_server_full_scale(handle):
+    """The served configuration the benchmark times: 1,024 resident 2048-bit
+    keys on 4 slots of 8,192 requests.  20,000 Poisson requests (half single
+    2048-bit powm, half CRT pairs) at 400 K req/s; GMP / Python int agree on
+    2,048 of them spread over the run, and 128 of those served alone (one in
+    flight) give the same bits as in the mix."""
+    keys = make_keys(1024, BITS, 99)
+    n = 20000
+    kinds, key_idx, bases, exps = _requests(keys, n, 11)
+    srv = Server(handle, keys, BITS, streams=4, max_batch=8192)
+    try:
+        res, lat, wall = srv.run_poisson(400000.0, kinds, key_idx, bases, exps, seed=12)
+        assert (lat > 0).all()
+        rows = np.unique(np.linspace(0, n - 1, num=2048).astype(int))
+        want = O.serving_expected(keys, kinds[rows], key_idx[rows], bases[rows], exps[rows], threads=THREADS)
+        assert_rows_equal(res[rows], want, "served in the mix")
+        alone_rows = rows[:: rows.size // 128][:128]
+        for i in alone_rows:
+            srv.submit(int(kinds[i]), int(key_idx[i]), bases[i], exps[i], int(i))
+            srv.drain()
+            tags, got, _ = srv.poll()
+            assert tags.tolist() == [i]
+            assert np.array_equal(got[0], res[i]), "request %d: alone differs from the mix" % i
+    finally:
+        srv.close()

@@ -590,12 +590,13 @@ XMP_RUN(13, 8, 38, 27)
 XMP_RUN(14, 16, 19, 27)
 XMP_RUN(15, 1, 5, 28)
 XMP_RUN(16, 1, 10, 28)
+XMP_RUN(17, 32, 20, 26)
+XMP_RUN(18, 64, 20, 26)
 #undef XMP_RUN
-// Variant 9 (2x55, which spills as a called function), 17 and 18 (whole-wave
-// layouts above 8192 bits) and the low-latency layouts 19..22 are not part of
-// the persistent kernel.
+// Variant 9 (2x55, which spills as a called function) and the low-latency
+// layouts 19..22 are not part of the persistent kernel.
 static_assert(kNumVariants == 23, "run_variant<> must list every batched variant in kVariants order");
-constexpr bool batched_variant(int i) { return i >= 0 && i < 17 && i != 9; }
+constexpr bool batched_variant(int i) { return i >= 0 && i < 19 && i != 9; }
 
 __global__ void __launch_bounds__(64, 2) powm_batch_kernel(BatchArgs ba) {
   extern __shared__ __attribute__((aligned(16))) uint32_t dyn_lds[];
@@ -624,6 +625,8 @@ __global__ void __launch_bounds__(64, 2) powm_batch_kernel(BatchArgs ba) {
       case 14: run_variant<14>(args, tile, dyn_lds); break;
       case 15: run_variant<15>(args, tile, dyn_lds); break;
       case 16: run_variant<16>(args, tile, dyn_lds); break;
+      case 17: run_variant<17>(args, tile, dyn_lds); break;
+      case 18: run_variant<18>(args, tile, dyn_lds); break;
       default: break;
     }
     wave_sync();

@@ -210,9 +210,21 @@ xmpError_t XMPAPI xmpIntegersDivMod(xmpHandle_t handle, xmpIntegers_t q, xmpInte
 }
 
 // ---- powm --------------------------------------------------------------
-xmpError_t XMPAPI xmpIntegersPowmAsync(xmpHandle_t handle, xmpIntegers_t out, const xmpIntegers_t a,
-                                       const xmpIntegers_t exp, const xmpIntegers_t mod, uint32_t count) {
-  XMP_TRACE("xmpIntegersPowmAsync");
+}  // extern "C"
+
+namespace xmp {
+
+// Plan and launch one powm call (reference powm.cu:250-307 and the scratch
+// planning of powm_operators.h:308-328).  The workspace holds R^2 mod m for
+// the moduli the call can touch, the division workspace of moduli above the
+// register division, the window table of one chunk of instances and, for an
+// in-place call with a fixed workspace, the output.  ws == nullptr: the
+// handle's grow-only scratch, limited by the policy's scratch limit (chunks
+// above 65,536 instances halve first, then the window narrows); otherwise the
+// fixed region, which is never grown (the window narrows to 3 bits first,
+// then chunks halve down to 64 instances, then the window to 2 bits).
+xmpError_t powm_run(xmpHandle_t handle, xmpIntegers_t out, xmpIntegers_t a, xmpIntegers_t exp, xmpIntegers_t mod,
+                    uint32_t count, const FixedWorkspace *ws) {
   xmpExecutionPolicy_t p = handle->policy;
   const int dev = handle->device;
   if (out->device != dev || a->device != dev || exp->device != dev || mod->device != dev)
@@ -241,31 +253,25 @@ xmpError_t XMPAPI xmpIntegersPowmAsync(xmpHandle_t handle, xmpIntegers_t out, co
     return wide ? powm_wide_table_words(wide, n, w) : powm_table_words(v, round_up(n, kStrideAlign), w);
   };
 
-  bool inplace = (out == a || out == exp || out == mod);
-  size_t obytes = inplace ? out_bytes(out) : 0;
-  uint32_t *obuf = out->limbs;
-  if (inplace) {
-    e = set_necessary_tmp_out(handle, obytes);
-    if (e != xmpErrorSuccess) return e;
-    obuf = (uint32_t *)handle->tmpOut;
-    if (p->indices[0] &&
-        hipMemcpyAsync(obuf, out->limbs, obytes, hipMemcpyDeviceToDevice, handle->stream) != hipSuccess)
-      return xmpErrorCuda;
-  }
-  // workspace: R^2 mod m per modulus, division workspace, window table per chunk
-  const uint32_t mstride = round_up(mod->count, kStrideAlign);
+  const bool inplace = (out == a || out == exp || out == mod);
+  const size_t obytes = inplace ? out_bytes(out) : 0;
+  // R^2 mod m for the moduli this call can reach: instance i < count reads
+  // modulus i % mod->count without a modulus index array, any of them with one
+  const uint32_t nmod = p->indices[3] ? mod->count : std::min(count, mod->count);
+  const uint32_t mstride = round_up(nmod, kStrideAlign);
   const size_t r2_words = (size_t)mod->nlimbs * mstride;
-  const size_t div_words = mod->nlimbs > kMaxRegDivisorLimbs ? (size_t)(2 * mod->nlimbs + 1) * round_up(mod->count, 256) : 0;
-  size_t limit = p->scratch_size_limit ? p->scratch_size_limit : handle->memorySize / 4 * 3;
-  size_t fixed = (r2_words + div_words) * sizeof(uint32_t);
+  const size_t div_words = mod->nlimbs > kMaxRegDivisorLimbs ? divmod_scratch_words(mod->nlimbs, nmod) : 0;
+  const size_t obuf_words = ws ? round_up((uint32_t)(obytes / sizeof(uint32_t)), 64) : 0;
+  const size_t limit = ws ? ws->bytes : (p->scratch_size_limit ? p->scratch_size_limit : handle->memorySize / 4 * 3);
+  const size_t fixed = (r2_words + div_words + obuf_words) * sizeof(uint32_t);
   uint32_t wbits = c.window_bits;
   uint32_t chunk = count;
-  // shrink the chunk (and then the window) until the table fits the limit
-  // (reference powm_operators.h:308-328 shrinks the window only)
-  while (true) {
-    size_t need = fixed + table_words(chunk, wbits) * sizeof(uint32_t);
-    if (need <= limit) break;
-    if (chunk > 65536) {
+  while (fixed + table_words(chunk, wbits) * sizeof(uint32_t) > limit) {
+    if (!ws && chunk > 65536) {
+      chunk = round_up(chunk / 2, kStrideAlign);
+    } else if (ws && wbits > 3) {
+      wbits--;
+    } else if (ws && chunk > kStrideAlign) {
       chunk = round_up(chunk / 2, kStrideAlign);
     } else if (wbits > 2) {
       wbits--;
@@ -273,15 +279,32 @@ xmpError_t XMPAPI xmpIntegersPowmAsync(xmpHandle_t handle, xmpIntegers_t out, co
       return xmpErrorIncreaseScratchLimit;
     }
   }
-  size_t need = fixed + table_words(chunk, wbits) * sizeof(uint32_t);
-  e = set_necessary_scratch(handle, need);
-  if (e != xmpErrorSuccess) return e;
-  uint32_t *r2 = (uint32_t *)handle->scratch;
+  const size_t need = fixed + table_words(chunk, wbits) * sizeof(uint32_t);
+  uint32_t *base;
+  uint32_t *obuf = out->limbs;
+  if (ws) {
+    base = (uint32_t *)ws->base;
+    if (inplace) obuf = base + r2_words + div_words;
+  } else {
+    e = set_necessary_scratch(handle, need);
+    if (e != xmpErrorSuccess) return e;
+    base = (uint32_t *)handle->scratch;
+    if (inplace) {
+      e = set_necessary_tmp_out(handle, obytes);
+      if (e != xmpErrorSuccess) return e;
+      obuf = (uint32_t *)handle->tmpOut;
+    }
+  }
+  // indexed outputs leave unselected instances untouched: start from out
+  if (inplace && p->indices[0] &&
+      hipMemcpyAsync(obuf, out->limbs, obytes, hipMemcpyDeviceToDevice, handle->stream) != hipSuccess)
+    return xmpErrorCuda;
+  uint32_t *r2 = base;
   uint32_t *divws = r2 + r2_words;
-  uint32_t *table = divws + div_words;
-  if ((wide ? launch_wide_r2(wide, r2, mstride, mod->limbs, mod->nlimbs, mod->stride, mod->count,
+  uint32_t *table = divws + div_words + obuf_words;
+  if ((wide ? launch_wide_r2(wide, r2, mstride, mod->limbs, mod->nlimbs, mod->stride, nmod,
                              div_words ? divws : nullptr, handle->stream)
-            : launch_montgomery_r2(c.variant, r2, mstride, mod->limbs, mod->nlimbs, mod->stride, mod->count,
+            : launch_montgomery_r2(c.variant, r2, mstride, mod->limbs, mod->nlimbs, mod->stride, nmod,
                                    div_words ? divws : nullptr, handle->stream)) != hipSuccess)
     return xmpErrorCuda;
   PowmArgs args;
@@ -304,6 +327,16 @@ xmpError_t XMPAPI xmpIntegersPowmAsync(xmpHandle_t handle, xmpIntegers_t out, co
     return xmpErrorCuda;
   XMP_CHECK_HIP();
   return xmpErrorSuccess;
+}
+
+}  // namespace xmp
+
+extern "C" {
+
+xmpError_t XMPAPI xmpIntegersPowmAsync(xmpHandle_t handle, xmpIntegers_t out, const xmpIntegers_t a,
+                                       const xmpIntegers_t exp, const xmpIntegers_t mod, uint32_t count) {
+  XMP_TRACE("xmpIntegersPowmAsync");
+  return powm_run(handle, out, a, exp, mod, count, nullptr);
 }
 
 xmpError_t XMPAPI xmpIntegersPowm(xmpHandle_t handle, xmpIntegers_t out, const xmpIntegers_t base,

@@ -81,6 +81,20 @@ bool powm_variant_batchable(int variant);
 // LDS staging bytes one wave of the variant needs
 size_t powm_variant_lds_bytes(int variant);
 
+// A powm call's workspace for callers that cannot allocate: a fixed device
+// region used as it is (never grown or freed by the call).
+struct FixedWorkspace {
+  uint8_t *base;
+  size_t bytes;
+};
+// xmpIntegersPowmAsync's validation, planning and launches on the handle's
+// stream.  ws == nullptr: the handle's grow-only scratch (allocating as
+// needed, within the policy's scratch limit); otherwise the fixed region, and
+// the call allocates nothing (xmpErrorIncreaseScratchLimit when even its
+// smallest plan does not fit).
+xmpError_t powm_run(xmpHandle_t handle, xmpIntegers_t out, xmpIntegers_t a, xmpIntegers_t exp, xmpIntegers_t mod,
+                    uint32_t count, const FixedWorkspace *ws);
+
 // Persistent mixed-size launch: tiles (class, tile) in queue order.
 struct BatchArgs {
   const PowmArgs *classes;        // per class (device array)

@@ -22,6 +22,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -105,6 +106,7 @@ struct _xmpServer_t {
     uint32_t *d_idx = nullptr;
     DevArray a_base, a_exp, a_out, b_c, b_cm, b_pow, b_prod, b_sum, b_out;
     uint32_t *table_a = nullptr, *table_b = nullptr;
+    uint32_t *div_ws = nullptr;   // division workspace for divisors above the register kernels (else null)
     std::vector<Request> reqs;
     uint32_t na = 0, nb = 0;
     bool busy = false;
@@ -120,6 +122,9 @@ struct _xmpServer_t {
   std::atomic<bool> running{false};
   std::thread worker;
   xmpError_t worker_error = xmpErrorSuccess;
+  // failure-handling test hook: with XMP_SERVER_FAIL_BATCH=k set at creation,
+  // the k-th batch launch reports an error after enqueuing its kind-0 work
+  uint32_t fail_batch = 0, launches = 0;
 };
 
 namespace {
@@ -193,6 +198,7 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
     return e;
   hipStream_t sb = sl.stream_b;
   if ((e = hipEventRecord(sl.fork, st)) || (e = hipStreamWaitEvent(sb, sl.fork, 0))) return e;
+  const bool inject_failure = s->fail_batch != 0 && ++s->launches == s->fail_batch;
   const size_t words = (size_t)cap * n32;
   // the imports read the rows this batch uses straight from the pinned buffer:
   // bases and exponents of the na powm requests, c of the nb CRT requests
@@ -219,12 +225,13 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
                            0, st)) != hipSuccess)
       return e;
   }
+  if (inject_failure) return hipErrorLaunchFailure;
   if (nb) {
     if ((e = launch_import(sl.b_c.data, sl.b_c.stride, n32, in_c, nb, n32, -1, -1, 0, 4, nullptr, 0, sb))) return e;
     // c mod p_k and c mod q_k in one launch over 2*nb instances
     OutOperand none{nullptr, 0, 0, nullptr};
     if ((e = launch_divmod_ws(none, outop(sl.b_cm), op(sl.b_c, d_cidx, 2 * nb), op(s->PQ, d_keyPQ, 2 * nb), 2 * nb,
-                              nullptr, sb)) != hipSuccess)
+                              sl.div_ws, sb)) != hipSuccess)
       return e;
     // mp = cm^dp mod p, mq = cm^dq mod q in one launch
     PowmArgs a;
@@ -245,7 +252,7 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
       return e;
     if ((e = launch_add(outop(sl.b_sum), op(sl.b_prod, d_lo, nb), op(sl.b_prod, d_hi, nb), nb, sb)) != hipSuccess)
       return e;
-    if ((e = launch_divmod_ws(none, outop(sl.b_out), op(sl.b_sum), op(s->N, d_keyB, nb), nb, nullptr, sb)) !=
+    if ((e = launch_divmod_ws(none, outop(sl.b_out), op(sl.b_sum), op(s->N, d_keyB, nb), nb, sl.div_ws, sb)) !=
         hipSuccess)
       return e;
     if ((e = launch_export((uint8_t *)(sl.g_out + words), sl.b_out.data, sl.b_out.stride, n32, nb, n32, -1, -1, 0, 4,
@@ -255,6 +262,25 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
   if ((e = hipEventRecord(sl.join, sb)) || (e = hipStreamWaitEvent(st, sl.join, 0))) return e;
   (void)h32;
   return hipEventRecord(sl.done, st);
+}
+
+// A batch whose launch or execution failed: wait until nothing of it is still
+// queued on the slot's streams, then count its requests as completed without
+// publishing results, and make the error sticky (Poll, Drain and RunPoisson
+// return it; no further batch is admitted).
+void retire_failed(xmpServer_t s, _xmpServer_t::Slot &sl) {
+  (void)hipStreamSynchronize(sl.stream);
+  (void)hipStreamSynchronize(sl.stream_b);
+  (void)hipGetLastError();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->worker_error = xmpErrorCuda;
+    s->completed += sl.reqs.size();
+    s->inflight -= (uint32_t)sl.reqs.size();
+  }
+  s->cv.notify_all();
+  sl.reqs.clear();
+  sl.busy = false;
 }
 
 void worker_loop(xmpServer_t s) {
@@ -268,6 +294,12 @@ void worker_loop(xmpServer_t s) {
       if (!sl.busy) continue;
       hipError_t q = hipEventQuery(sl.done);
       if (q == hipErrorNotReady) continue;
+      if (q != hipSuccess) {
+        // the batch failed on the device: its results are not published
+        retire_failed(s, sl);
+        progressed = true;
+        continue;
+      }
       auto now = Clock::now();
       std::vector<Done> out;
       out.reserve(sl.reqs.size());
@@ -283,7 +315,6 @@ void worker_loop(xmpServer_t s) {
       }
       {
         std::lock_guard<std::mutex> lk(s->mu);
-        if (q != hipSuccess) s->worker_error = xmpErrorCuda;
         for (auto &d : out) s->done.push_back(std::move(d));
         s->completed += sl.reqs.size();
         s->inflight -= (uint32_t)sl.reqs.size();
@@ -294,11 +325,12 @@ void worker_loop(xmpServer_t s) {
       progressed = true;
     }
     // 2. admit pending requests into a free slot, within the HBM budget
+    // (nothing more is admitted once a batch has failed: the error is sticky)
     for (auto &sl : s->slots) {
       if (sl.busy) continue;
       {
         std::lock_guard<std::mutex> lk(s->mu);
-        if (s->pending.empty()) break;
+        if (s->pending.empty() || s->worker_error != xmpErrorSuccess) break;
         uint32_t budget = s->max_inflight > s->inflight ? s->max_inflight - s->inflight : 0;
         uint32_t take = std::min<uint32_t>({sl.cap, budget, (uint32_t)s->pending.size()});
         if (take == 0) break;
@@ -324,18 +356,20 @@ void worker_loop(xmpServer_t s) {
       }
       for (uint32_t j = 0; j < sl.nb; j++)
         std::memcpy(sl.h_in + 2 * words + (size_t)j * n32, sl.reqs[sl.na + j].base.data(), n32 * 4);
+      progressed = true;
       if (launch_batch(s, sl) != hipSuccess) {
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->worker_error = xmpErrorCuda;
+        // part of the batch may be enqueued on either stream: wait for both
+        // before the slot's buffers are reused, and fail its requests
+        retire_failed(s, sl);
+        continue;
       }
       sl.busy = true;
-      progressed = true;
     }
     if (!s->running.load()) {
       bool any_busy = false;
       for (auto &sl : s->slots) any_busy |= sl.busy;
       std::lock_guard<std::mutex> lk(s->mu);
-      if (!any_busy && s->pending.empty()) break;
+      if (!any_busy && (s->pending.empty() || s->worker_error != xmpErrorSuccess)) break;
     }
     if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
   }
@@ -351,6 +385,7 @@ void free_server(xmpServer_t s) {
       if (a->data) h->df(a->data);
     if (sl.table_a) h->df(sl.table_a);
     if (sl.table_b) h->df(sl.table_b);
+    if (sl.div_ws) h->df(sl.div_ws);
     if (sl.d_idx) h->df(sl.d_idx);
     if (sl.h_in) (void)hipHostFree(sl.h_in);
     if (sl.h_idx) (void)hipHostFree(sl.h_idx);
@@ -394,6 +429,7 @@ xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const
   s->n32 = bits / 32;
   s->h32 = bits / 64;
   s->nkeys = nkeys;
+  if (const char *f = std::getenv("XMP_SERVER_FAIL_BATCH")) s->fail_batch = (uint32_t)std::strtoul(f, nullptr, 10);
   xmpError_t e = xmpErrorSuccess;
   auto fail = [&](xmpError_t err) {
     free_server(s);
@@ -470,6 +506,11 @@ xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const
     sl.table_a = (uint32_t *)handle->da(sizeof(uint32_t) * tab_a);
     sl.table_b = (uint32_t *)handle->da(sizeof(uint32_t) * tab_b);
     if (!sl.d_idx || !sl.table_a || !sl.table_b) return fail(xmpErrorInvalidCudaMalloc);
+    // the CRT chain's two divisions (c mod p/q over 2*cap instances, the sum
+    // mod n over cap) share one workspace, needed above 128-limb divisors
+    const size_t div_words = std::max(divmod_scratch_words(s->h32, 2 * cap), divmod_scratch_words(s->n32, cap));
+    if (div_words && !(sl.div_ws = (uint32_t *)handle->da(sizeof(uint32_t) * div_words)))
+      return fail(xmpErrorInvalidCudaMalloc);
     if (alloc_array(handle, sl.a_base, s->n32, cap) || alloc_array(handle, sl.a_exp, s->n32, cap) ||
         alloc_array(handle, sl.a_out, s->n32, cap) || alloc_array(handle, sl.b_c, s->n32, cap) ||
         alloc_array(handle, sl.b_cm, s->h32, 2 * cap) || alloc_array(handle, sl.b_pow, s->h32, 2 * cap) ||

@@ -57,3 +57,26 @@ def ordered_pair(seed, count, limbs):
     a2, b2 = a.copy(), b.copy()
     a2[swap], b2[swap] = b[swap], a[swap]
     return a2, b2
+
+
+# The mixed-size draw: modulus bit-lengths with their weights (exponents as
+# long as the modulus).
+MIXED_SIZES = (512, 1024, 2048, 3072, 4096, 8192)
+MIXED_WEIGHTS = (0.10, 0.20, 0.35, 0.15, 0.15, 0.05)
+
+
+def mixed_counts(total, seed):
+    """Class sizes of a mixed batch: `total` draws over MIXED_SIZES with MIXED_WEIGHTS."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    draws = rng.choice(len(MIXED_SIZES), size=total, p=MIXED_WEIGHTS)
+    return [int((draws == k).sum()) for k in range(len(MIXED_SIZES))]
+
+
+def powm_class(seed, n, bits, exp_bits=None):
+    """One precision class of powm inputs: odd moduli (top bit set), bases
+    below them, uniform exponents of exp_bits (default: bits)."""
+    limbs = bits // 32
+    m = odd_moduli(seed, n, limbs)
+    b = below(seed + 1, m)
+    e = uniform(seed + 2, n, (exp_bits or bits) // 32)
+    return m, b, e
