@@ -21,12 +21,12 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# The serving loop runs each batch slot on two streams (4 slots, 8 streams).
-# HIP maps streams onto this many hardware queues per process (default 4), so
-# raise it to at least 8 for the streams to run concurrently; it must be set
-# before the first HIP call.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+# The serving loop runs each batch slot on two streams, four with reserved CUs
+# (4 slots: 8 or 16 streams).  HIP maps streams onto this many hardware queues
+# per process (default 4), so raise it to 16 for the streams to run
+# concurrently; it must be set before the first HIP call.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 import xmp_amd  # noqa: E402
 from xmp_amd import hip, inputs  # noqa: E402
@@ -139,9 +139,14 @@ def run_mixed(h, stream, total, seed, reps=2):
             "batched_over_plain": round(plain_ms / batched_ms, 3)}, samples
 
 
-def run_serving(h, nkeys, requests, seed):
-    """Serving loop over `nkeys` resident 2048-bit keys: saturation throughput,
-    then Poisson arrivals at 50% and 80% of it; p50/p99 latency per rate."""
+SERVE_SLOTS, SERVE_BATCH = 4, 8192
+
+
+def run_serving(h, nkeys, requests, seed, reserved_cus=0):
+    """Serving loop over `nkeys` resident 2048-bit keys.  Saturation is the
+    throughput achieved under Poisson arrivals offered far above capacity
+    (3 M req/s); then Poisson arrivals at fixed 400 K and 800 K req/s and at
+    50% and 80% of the saturation, each with p50/p99 latency."""
     from xmp_amd.ext import Server, make_keys
     keys = make_keys(nkeys, 2048, seed)
     rng = np.random.default_rng(seed)
@@ -151,23 +156,26 @@ def run_serving(h, nkeys, requests, seed):
     nmat = keys["n"][key_idx]
     bases[:, -1] = (bases[:, -1].astype(np.uint64) % nmat[:, -1].astype(np.uint64)).astype(np.uint32)
     exps = inputs.uniform(seed + 2, requests, 64)
-    # 4 slots of 8,192 requests (8 hardware queues): in a sweep of slot shapes the best
-    # saturation of the 4-slot shapes (948 K req/s) and p50/p99 16.8/32.1 ms at 600 K
-    # req/s; 8 slots reached 1.03 M req/s but need 16 hardware queues
-    srv = Server(h, keys, 2048, streams=4, max_batch=8192)
+    srv = Server(h, keys, 2048, streams=SERVE_SLOTS, max_batch=SERVE_BATCH, reserved_cus=reserved_cus)
     out = {"keys": nkeys, "requests": requests, "mix": "50% 2048-bit powm, 50% CRT pairs of 1024-bit moduli",
-           "streams": 4, "max_batch": 8192}
+           "streams": SERVE_SLOTS, "max_batch": SERVE_BATCH, "reserved_cus": reserved_cus}
+
+    def point(rate, seed_):
+        res, lat, wall = srv.run_poisson(rate, kinds, key_idx, bases, exps, seed=seed_)
+        return res, {"offered_req_per_s": round(rate, 1), "achieved_req_per_s": round(requests / wall, 1),
+                     "achieved_frac": round(requests / wall / rate, 4),
+                     "p50_ms": round(float(np.percentile(lat, 50)) / 1e3, 3),
+                     "p99_ms": round(float(np.percentile(lat, 99)) / 1e3, 3)}
     try:
         srv.run_poisson(1e9, kinds[:4096], key_idx[:4096], bases[:4096], exps[:4096], seed=seed)   # warm-up
-        res, lat, wall = srv.run_poisson(1e9, kinds, key_idx, bases, exps, seed=seed)
-        sat = requests / wall
-        out["saturation_req_per_s"] = round(sat, 1)
+        res, sat = point(3e6, seed)
+        out["saturation_req_per_s"] = sat["achieved_req_per_s"]
+        out["saturation_run"] = sat
+        for rate in (400000.0, 800000.0):
+            res, out["rate_%dK" % int(rate / 1000)] = point(rate, seed + 1)
         for frac in (0.5, 0.8):
-            res, lat, wall = srv.run_poisson(frac * sat, kinds, key_idx, bases, exps, seed=seed + 1)
-            out["rate_%d%%" % int(frac * 100)] = {
-                "offered_req_per_s": round(frac * sat, 1), "achieved_req_per_s": round(requests / wall, 1),
-                "p50_ms": round(float(np.percentile(lat, 50)) / 1e3, 3),
-                "p99_ms": round(float(np.percentile(lat, 99)) / 1e3, 3)}
+            res, out["rate_%d%%" % int(frac * 100)] = point(frac * out["saturation_req_per_s"], seed + 1)
+        out["stats"] = {k: v for k, v in srv.stats().items() if k in ("batches", "peak_batch", "peak_inflight")}
     finally:
         srv.close()
     sample = np.linspace(0, requests - 1, num=64).astype(int)
@@ -180,6 +188,11 @@ def run_only(args):
     stream = hip.stream_create()
     h.set_stream(stream)
     lib = xmp_amd.lib()
+    if args.only == "serve":
+        res, _ = run_serving(h, 1024, 200000, 99, reserved_cus=args.serve_reserved_cus)
+        print(json.dumps(res), flush=True)
+        h.close()
+        return
     if args.only == "mixed":
         res, _ = run_mixed(h, stream, 262144, 777, reps=max(1, args.steps))
         print(json.dumps(res), flush=True)
@@ -197,7 +210,14 @@ def run_only(args):
         A2, B2 = h.from_array(a2), h.from_array(b2)
         P2 = h.integers(4096, n2)
         fn = lambda: lib.xmpIntegersMulAsync(h.h, P2.x, A2.x, B2.x, n2)  # noqa: E731
-        what = "mul_lanes_kernel<2,37,28> 1,048,576 x 2048x2048 -> 4096-bit (4096 canonical MACs and 1024 B per product)"
+        what = "mul_cols_kernel<74,74,28> 1,048,576 x 2048x2048 -> 4096-bit (4096 canonical MACs and 1024 B per product)"
+    elif args.only == "sqr":
+        n2 = 1 << 20
+        a2, _ = inputs.ordered_pair(4242, n2, 64)
+        A2 = h.from_array(a2)
+        P2 = h.integers(4096, n2)
+        fn = lambda: lib.xmpIntegersMulAsync(h.h, P2.x, A2.x, A2.x, n2)  # noqa: E731
+        what = "mul_cols_kernel<74,74,28,square> 1,048,576 x 2048-bit squares -> 4096-bit"
     elif args.only == "add":
         n2 = 1 << 20
         a2, b2 = inputs.ordered_pair(4242, n2, 64)
@@ -226,7 +246,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["powm", "add", "mul", "copy", "mixed"],
+    ap.add_argument("--serve-reserved-cus", type=int, default=0,
+                    help="CUs the serving loop keeps for its short kernels (0 = none)")
+    ap.add_argument("--only", choices=["powm", "add", "mul", "sqr", "copy", "mixed", "serve"],
                     help="run just one kernel (for counter profiling): the 2048-bit powm kernel with resident "
                          "constants, the 2048-bit add, or a plain 256 MiB import copy for counter calibration")
     args = ap.parse_args()
@@ -355,6 +377,7 @@ def main():
         add = lambda: lib.xmpIntegersAddAsync(h.h, S2.x, A2.x, B2.x, n2)  # noqa: E731
         sub = lambda: lib.xmpIntegersSubAsync(h.h, S2.x, A2.x, B2.x, n2)  # noqa: E731
         mul = lambda: lib.xmpIntegersMulAsync(h.h, P2.x, A2.x, B2.x, n2)  # noqa: E731
+        sqr = lambda: lib.xmpIntegersMulAsync(h.h, P2.x, A2.x, A2.x, n2)  # noqa: E731
         xor = lambda: lib.xmpIntegersXorAsync(h.h, S2.x, A2.x, B2.x, n2)  # noqa: E731
         add()
         hip.stream_sync(stream)
@@ -366,6 +389,9 @@ def main():
         mul()
         hip.stream_sync(stream)
         mul_ms = time_async(mul, stream, 10)
+        sqr()
+        hip.stream_sync(stream)
+        sqr_ms = time_async(sqr, stream, 10)
         extra["add_2048"] = {"pairs": n2, "ms": round(add_ms, 4), "GB_per_s": round(768 * n2 / add_ms / 1e6, 1),
                              "hbm_frac": round(768 * n2 / add_ms / 1e6 / HBM_PEAK_GBS, 4)}
         extra["sub_2048"] = {"pairs": n2, "ms": round(sub_ms, 4), "GB_per_s": round(768 * n2 / sub_ms / 1e6, 1),
@@ -375,7 +401,11 @@ def main():
         extra["mul_2048x2048"] = {"pairs": n2, "ms": round(mul_ms, 4), "mul_per_s": round(n2 / mul_ms * 1e3, 1),
                                   "GB_per_s": round(1024 * n2 / mul_ms / 1e6, 1),
                                   "TOP_per_s": round(2 * 4096 * n2 / mul_ms / 1e9, 3)}
-        log("add %.3f ms  sub %.3f ms  xor %.3f ms  mul %.3f ms" % (add_ms, sub_ms, xor_ms, mul_ms))
+        extra["sqr_2048"] = {"count": n2, "ms": round(sqr_ms, 4), "sqr_per_s": round(n2 / sqr_ms * 1e3, 1),
+                             "GB_per_s": round(768 * n2 / sqr_ms / 1e6, 1),
+                             "TOP_per_s_as_products": round(2 * 4096 * n2 / sqr_ms / 1e9, 3)}
+        log("add %.3f ms  sub %.3f ms  xor %.3f ms  mul %.3f ms  sqr %.3f ms" % (add_ms, sub_ms, xor_ms, mul_ms,
+                                                                             sqr_ms))
         cpu_work += [("add_2048", "GO_ADD", n2, 64, a2, b2, None), ("sub_2048", "GO_SUB", n2, 64, a2, b2, None),
                      ("mul_2048x2048", "GO_MUL", n2, 128, a2, b2, None)]
         del A2, B2, S2, P2
@@ -415,7 +445,7 @@ def main():
         log("mixed batch: batched %.1f ms, plain %.1f ms (x%.2f)" % (
             mixed["batched_ms"], mixed["plain_ms"], mixed["batched_over_plain"]))
         # ---- serving: 1024 resident keys, Poisson arrivals
-        serving, serve_sample = run_serving(h, 1024, 200000, 99)
+        serving, serve_sample = run_serving(h, 1024, 200000, 99, reserved_cus=args.serve_reserved_cus)
         extra["serving"] = serving
         log("serving: %s" % json.dumps(serving))
 

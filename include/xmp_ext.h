@@ -71,18 +71,34 @@ xmpError_t XMPAPI xmpBatchLaunchPlainAsync(xmpHandle_t handle, xmpBatch_t batch)
 typedef struct _xmpServer_t *xmpServer_t;
 
 typedef struct {
-  uint32_t streams;    /* batch slots in flight at once (each on two streams: the single powm
-                          requests and the CRT requests of a batch run concurrently; set
-                          GPU_MAX_HW_QUEUES >= 2 * streams for all of them to overlap) */
-  uint32_t max_batch;  /* requests per batch slot */
-  size_t hbm_budget;   /* device bytes for in-flight request state; 0 = streams * max_batch requests */
+  uint32_t streams;       /* batch slots in flight at once (each on two streams: the single powm
+                             requests and the CRT requests of a batch run concurrently; set
+                             GPU_MAX_HW_QUEUES >= 2 * streams for all of them to overlap) */
+  uint32_t max_batch;     /* requests per batch slot (lowered to fit the budget) */
+  size_t hbm_budget;      /* device bytes for the slots' request state and the requests in flight;
+                             0 = half of the device memory free at creation */
+  uint32_t reserved_cus;  /* CUs kept for the short kernels (imports, the CRT chain's divisions,
+                             multiply and add, exports) on two more streams per slot, which the
+                             exponentiations do not use; 0 = none (set GPU_MAX_HW_QUEUES >=
+                             4 * streams when used) */
 } xmpServerConfig_t;
+
+typedef struct {
+  uint64_t batches;          /* batches launched */
+  uint32_t slot_capacity;    /* requests per batch after the budget */
+  uint32_t max_inflight;     /* requests the budget lets be in flight at once */
+  uint32_t peak_inflight;    /* most requests that were in flight at once */
+  uint32_t peak_batch;       /* largest batch launched */
+  size_t bytes_per_request;  /* device bytes one request in flight holds */
+  size_t hbm_budget;         /* the budget in force */
+} xmpServerStats_t;
 
 /* Keys (host arrays, `nkeys` each, 32-bit little-endian limbs, compact): n
  * (bits), p and q (bits/2, odd, coprime), dp and dq (bits/2), cp and cq
  * (bits, with cp = 1 mod p, 0 mod q and cq = 0 mod p, 1 mod q).  They stay
  * resident on the device with their Montgomery constants.  A scheduler
- * thread batches arriving requests into free slots within the HBM budget. */
+ * thread batches arriving requests into free slots within the HBM budget and
+ * a completer thread publishes each finished batch's results. */
 xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const xmpServerConfig_t *config,
                                   uint32_t bits, uint32_t nkeys, const uint32_t *n, const uint32_t *p,
                                   const uint32_t *q, const uint32_t *dp, const uint32_t *dq, const uint32_t *cp,
@@ -97,6 +113,8 @@ xmpError_t XMPAPI xmpServerPoll(xmpServer_t server, uint32_t max, uint64_t *tags
                                 double *latency_us, uint32_t *n_out);
 /* wait until every submitted request has completed */
 xmpError_t XMPAPI xmpServerDrain(xmpServer_t server);
+/* admission and budget counters since creation */
+xmpError_t XMPAPI xmpServerGetStats(xmpServer_t server, xmpServerStats_t *stats);
 /* load generator: `requests` requests arriving as a Poisson process of the
  * given rate (seeded); request i is (kinds[i], keys[i], bases[i], exps[i]).
  * Fills results and per-request latency (indexed by request) and the wall time. */

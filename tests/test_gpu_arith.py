@@ -115,7 +115,9 @@ def test_add_in_place(handle):
                                                   (2048, 1024, 2080), (4096, 2048, 4096), (8192, 8192, 16384),
                                                   (8192, 4096, 8192), (12288, 64, 12352), (16384, 16384, 32768),
                                                   (16384, 16384, 16384), (32, 2048, 2080),
-                                                  (32768, 32768, 65536), (32768, 16384, 32768)])
+                                                  (32768, 32768, 65536), (32768, 16384, 32768),
+                                                  (1024, 544, 1568), (2048, 544, 2592), (1600, 1600, 3200),
+                                                  (1056, 1056, 2112), (992, 992, 1984)])
 def test_mul(handle, bits_a, bits_b, bits_p):
     n = 500
     a, b = _pair(51, n, bits_a // 32, bits_b // 32)
@@ -153,6 +155,20 @@ def test_mul_all_ones_and_indices(handle, bits):
     assert_rows_equal(P.get(), want, "indexed mul")
 
 
+@pytest.mark.parametrize("bits_a,bits_p", [(544, 1088), (992, 1984), (1024, 2048), (1024, 1024), (1024, 2080),
+                                           (1568, 3136), (1600, 3200), (2048, 4096), (2048, 2048), (2048, 4160),
+                                           (4096, 8192)])
+def test_sqr(handle, bits_a, bits_p):
+    """mul(a, a) takes the square kernels (each product of two different
+    digits once, doubled); truncated and zero-extended outputs."""
+    n = 300
+    a = inputs.uniform(91 + bits_a, n, bits_a // 32)
+    A = handle.from_array(a)
+    P = handle.integers(bits_p, n)
+    handle.mul(P, A, A)
+    assert_rows_equal(P.get(), gmp(O.GO_MUL, n, bits_p // 32, a, a), "square %d -> %d bits" % (bits_a, bits_p))
+
+
 def test_mul_full_size_sample(handle):
     """1,048,576 x 2048-bit products (the benchmark shape): a sample of rows
     against GMP, and a * 1 == a on every row."""
@@ -163,6 +179,9 @@ def test_mul_full_size_sample(handle):
     handle.mul(P, A, B)
     sample = np.arange(0, n, 1009)
     assert_rows_equal(P.get()[sample], gmp(O.GO_MUL, sample.size, 128, a[sample], b[sample]), "mul sample")
+    # the same operands squared: a sample against GMP, and (a^2 == a*a) where b = a
+    handle.mul(P, A, A)
+    assert_rows_equal(P.get()[sample], gmp(O.GO_MUL, sample.size, 128, a[sample], a[sample]), "square sample")
     one = np.zeros((1, 64), np.uint32)
     one[0, 0] = 1
     handle.mul(P, A, handle.from_array(one))

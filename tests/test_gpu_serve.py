@@ -132,3 +132,52 @@ def test_server_full_scale(handle):
             assert np.array_equal(got[0], res[i]), "request %d: alone differs from the mix" % i
     finally:
         srv.close()
+
+
+def test_server_hbm_budget_binds_under_load(handle):
+    """The HBM budget: by default half the device memory free at creation
+    (it does not bind at these sizes); a budget of 100 requests' bytes shrinks
+    two slots of 256 to 50 each, and under arrivals far above capacity the
+    scheduler fills it but never exceeds it.  Every result is still exact."""
+    keys = make_keys(16, BITS, 79)
+    kinds, key_idx, bases, exps = _requests(keys, 2000, 13)
+    want = O.serving_expected(keys, kinds, key_idx, bases, exps, threads=THREADS)
+    srv = Server(handle, keys, BITS, streams=2, max_batch=256)
+    try:
+        st = srv.stats()
+        assert st["hbm_budget"] > 0 and st["slot_capacity"] == 256 and st["max_inflight"] == 512
+        bpr = st["bytes_per_request"]
+    finally:
+        srv.close()
+    srv = Server(handle, keys, BITS, streams=2, max_batch=256, hbm_budget=100 * bpr)
+    try:
+        st = srv.stats()
+        assert st["slot_capacity"] == 50 and 90 <= st["max_inflight"] <= 100, st
+        res, lat, wall = srv.run_poisson(1e6, kinds, key_idx, bases, exps, seed=14)
+        st = srv.stats()
+        assert st["peak_inflight"] == st["max_inflight"], st
+        assert st["peak_batch"] <= st["slot_capacity"], st
+        assert st["batches"] >= 2000 // 50
+    finally:
+        srv.close()
+    assert_rows_equal(res, want, "served within a 100-request budget")
+
+
+def test_server_reserved_cus(handle):
+    """Short kernels on CU-masked streams (reserved_cus): the same results
+    alone and in a Poisson mix."""
+    keys = make_keys(16, BITS, 80)
+    kinds, key_idx, bases, exps = _requests(keys, 400, 15)
+    want = O.serving_expected(keys, kinds, key_idx, bases, exps, threads=THREADS)
+    srv = Server(handle, keys, BITS, streams=2, max_batch=128, reserved_cus=8)
+    try:
+        for i in range(8):
+            srv.submit(int(kinds[i]), int(key_idx[i]), bases[i], exps[i], i)
+            srv.drain()
+            tags, res1, _ = srv.poll()
+            assert tags.tolist() == [i]
+            assert np.array_equal(res1[0], want[i])
+        res, lat, wall = srv.run_poisson(300000.0, kinds, key_idx, bases, exps, seed=16)
+    finally:
+        srv.close()
+    assert_rows_equal(res, want, "served with reserved CUs")

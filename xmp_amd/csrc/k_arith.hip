@@ -14,6 +14,8 @@
 // product is formed row by row in 64-bit "lazy" accumulators: each partial
 // product is one v_mad_u64_u32 and carries are resolved once per retired
 // digit, because up to 255 products of 56 bits fit in 64 bits.
+#include <utility>
+
 #include "devmp.hpp"
 #include "kernels.hpp"
 #include "lanes.hpp"
@@ -281,6 +283,172 @@ hipError_t launch_mul_lanes(OutOperand p, Operand a, Operand b, uint32_t count, 
   return hipGetLastError();
 }
 
+// One limb of a row of the interleaved layout at a lane's byte offset, as a
+// raw buffer access: the row's resource is built in scalar registers and the
+// lane's offset is one loop-invariant VGPR, so walking the rows of an
+// operand costs no vector address arithmetic.  Offsets stay below 4 GiB
+// (instance slots below 2^30, checked by the launcher).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const uint32_t *row) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(row), 0, -1, 0x00020000);
+}
+__device__ __forceinline__ uint32_t ld_row(const uint32_t *row, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(row_rsrc(row), (int)off, 0, 0);
+}
+__device__ __forceinline__ void st_row(uint32_t *row, uint32_t off, uint32_t v) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, row_rsrc(row), (int)off, 0, 0);
+}
+
+// Product scanning with one lane per product (up to 74 radix-2^28 digits per
+// operand).  Both operands' digits sit in the lane's registers and the
+// product's columns are formed in order: column c is the lazy 64-bit sum of
+// every a_i b_j with i + j = c plus the previous column's carry (74 products
+// of < 2^56 and a carry < 2^36 stay below 2^63), its low D bits are the
+// product's digit c, and each output limb is packed and stored as soon as
+// its second digit is final.  Every register index is a compile-time
+// constant (the columns are unrolled), so there is no cross-lane traffic, no
+// window shift and no per-row packing branch: the instruction stream is the
+// multiply-adds plus two instructions per column and two per output limb.
+// A square (SQR) adds each product of two different digits once, against
+// the doubled digit (a_i (2 a_j) for i < j, < 2^57 each), and the squares
+// a_k^2 on the diagonal: about half the multiply-adds.
+template <int NA, int NB, int D, bool SQR>
+struct ColMul {
+  static constexpr uint32_t MASK = (1u << D) - 1;
+  static constexpr int NC = NA + NB - 1;                  // product columns
+  static constexpr int NOUT = (D * (NC + 1) + 31) / 32;   // limbs the product's NC+1 digits reach
+  static_assert(NA <= 74 && NB <= 74, "column sums must stay below 2^63");
+  static_assert(!SQR || NA == NB, "a square has one operand");
+
+  const uint32_t (&a)[NA];
+  const uint32_t (&b)[NB];   // the multiplier's digits; for a square the doubled digits 2 a_j
+  uint32_t *out;             // row of the next limb to store (uniform; limbs are stored in order)
+  size_t stride;
+  uint32_t inst;             // the product's slot within a row
+  uint32_t len;
+  uint32_t dig[NC + 2];   // the product's digits; dig[NC + 1] = 0 pads the last partial limb
+
+  // the product a_i b_j of column i + j (for a square only i < j)
+  template <int I, int J>
+  __device__ __forceinline__ void term(uint64_t &acc) const {
+    if constexpr (!SQR || I < J) acc += (uint64_t)a[I] * b[J];
+  }
+
+  // column C: the products a_i b_{C-i} (for a square, then a_{C/2}^2)
+  template <int C, int... I>
+  __device__ __forceinline__ uint64_t col(uint64_t acc, std::integer_sequence<int, I...>) const {
+    constexpr int LO = C - NB + 1 > 0 ? C - NB + 1 : 0;
+    (term<LO + I, C - LO - I>(acc), ...);
+    if constexpr (SQR && C % 2 == 0) acc += (uint64_t)a[C / 2] * a[C / 2];
+    return acc;
+  }
+
+  // the limb that becomes complete with digit K (limb w needs digits
+  // floor(32w/D) and the next one; D = 28 < 32 so at most one per digit)
+  template <int K>
+  __device__ __forceinline__ bool emit() {
+    constexpr int W = (D * K + 31) / 32 - 1;   // last limb whose second digit is K ...
+    constexpr int K0 = W >= 0 ? (32 * W) / D : 0;
+    if constexpr (W < 0 || K0 + 1 != K) {
+      return false;
+    } else {
+      constexpr int S = 32 * W - D * K0;   // 0 .. D-4
+      static_assert(S + 32 <= 2 * D, "a limb spans at most two digits");
+      if ((uint32_t)W >= len) return true;
+      static_assert(K0 + 1 <= NC + 1, "digit index");
+      const uint32_t v = (dig[K0] >> S) | (dig[K0 + 1] << (D - S));
+      st_row(out, inst * 4u, v);
+      out += stride;
+      return (uint32_t)W + 1 >= len;   // every limb written: stop
+    }
+  }
+
+  template <int C>
+  __device__ __forceinline__ bool step(uint64_t &acc) {
+    constexpr int LO = C - NB + 1 > 0 ? C - NB + 1 : 0;
+    constexpr int HI = C < NA - 1 ? C : NA - 1;
+    acc = col<C>(acc, std::make_integer_sequence<int, HI - LO + 1>());
+    dig[C] = (uint32_t)acc & MASK;
+    acc >>= D;
+    return emit<C>();
+  }
+
+  template <int... C>
+  __device__ __forceinline__ bool run(std::integer_sequence<int, C...>) {
+    uint64_t acc = 0;
+    // stops at the first column after which every output limb is written
+    const bool done = (step<C>(acc) || ...);
+    if (done) return true;
+    dig[NC] = (uint32_t)acc;   // the top digit (< 2^D: the product has NC+1 digits)
+    dig[NC + 1] = 0;
+    return emit<NC>() || emit<NC + 1>();
+  }
+};
+
+// digits 0 .. N-1 (radix 2^D) of an integer of len 32-bit limbs, lane-private
+// (the lane's digits start at bit 0, so every shift is a constant); limbs at
+// or above len read as zero
+template <int N, int D>
+__device__ __forceinline__ void load_digits(uint32_t (&d)[N], const uint32_t *data, uint32_t stride, uint32_t inst,
+                                            uint32_t len) {
+  constexpr int NL = (D * N + 31) / 32;
+  uint32_t w[NL + 1];
+  // the row pointer is uniform (scalar) and the instance a 32-bit lane
+  // offset, so a load needs no vector address arithmetic
+  const uint32_t *row = data;
+  const uint32_t off = inst * 4u;
+#pragma unroll
+  for (int j = 0; j < NL; j++) {
+    w[j] = 0;
+    if ((uint32_t)j < len) w[j] = ld_row(row, off);   // uniform branch
+    row += stride;
+  }
+  w[NL] = 0;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const int bit = D * k, j = bit >> 5, s = bit & 31;
+    d[k] = (s + D <= 32 ? (w[j] >> s) : __builtin_amdgcn_alignbit(w[j + 1], w[j], s)) & ((1u << D) - 1);
+  }
+}
+
+template <int NA, int NB, int D, bool SQR>
+__global__ void __launch_bounds__(64, 3) mul_cols_kernel(OutOperand p, Operand a, Operand b, uint32_t count) {
+  // Lanes past `count` (in the last wave only) take instance count-1, which
+  // is in the same wave: they compute and store exactly its limbs, so their
+  // stores need no mask (the duplicate writes carry identical values).
+  const uint32_t t = blockIdx.x * 64 + threadIdx.x;
+  const uint32_t i = t < count ? t : count - 1;
+  const uint32_t ai = in_index(a, i);
+  const uint32_t pi = out_index(p, i);
+  // instance slots are below 2^30 (the launcher checks the strides), so a
+  // slot is a 32-bit byte offset from a row
+  __builtin_assume(ai < (1u << 30));
+  __builtin_assume(pi < (1u << 30));
+  uint32_t da[NA], db[NB];
+  load_digits<NA, D>(da, a.data, a.stride, ai, a.len);
+  if constexpr (SQR) {
+#pragma unroll
+    for (int k = 0; k < NB; k++) db[k] = da[k] << 1;
+  } else {
+    const uint32_t bi = in_index(b, i);
+    __builtin_assume(bi < (1u << 30));
+    load_digits<NB, D>(db, b.data, b.stride, bi, b.len);
+  }
+  using CM = ColMul<NA, NB, D, SQR>;
+  CM m{da, db, p.data, p.stride, pi, p.len, {}};
+  if (m.run(std::make_integer_sequence<int, CM::NC>())) return;
+  // past the product: zero limbs
+  for (uint32_t w = (uint32_t)CM::NOUT; w < p.len; w++) p.data[(size_t)w * p.stride + pi] = 0u;
+}
+
+template <int NA, int NB, bool SQR>
+hipError_t launch_mul_cols(OutOperand p, Operand a, Operand b, uint32_t count, hipStream_t stream) {
+  // instance slots must stay below 2^30 (every slot is below its stride)
+  if (a.stride >= (1u << 30) || b.stride >= (1u << 30) || p.stride >= (1u << 30)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((mul_cols_kernel<NA, NB, 28, SQR>), dim3(div_round_up(count, 64)), dim3(64), 0, stream, p, a, b,
+                     count);
+  return hipGetLastError();
+}
+
 // Generic fallback for any sizes: schoolbook with 32-bit limbs, the row
 // accumulator kept in the output array itself (each lane owns its column).
 __global__ void __launch_bounds__(256) mul_general_kernel(OutOperand p, Operand a, Operand b, uint32_t count) {
@@ -335,6 +503,13 @@ hipError_t launch_sub(OutOperand d, Operand a, Operand b, uint32_t count, hipStr
 }
 
 hipError_t launch_mul(OutOperand p, Operand a, Operand b, uint32_t count, hipStream_t stream) {
+  // one lane per product with both operands in registers (product scanning)
+  // for 513-1024-bit by 513-1024-bit and 1537-2048-bit by 513-2048-bit
+  if (a.len > 16 && a.len <= 32 && b.len > 16) return launch_mul_cols<37, 37, false>(p, a, b, count, stream);
+  if (a.len > 48 && a.len <= 64 && b.len > 16) {
+    if (b.len <= 32) return launch_mul_cols<74, 37, false>(p, a, b, count, stream);
+    return launch_mul_cols<74, 74, false>(p, a, b, count, stream);
+  }
   // register-resident row kernels, the longer operand (a) as the multiplicand:
   // one lane per instance up to 1024 bits, then lane groups; b.len <= a.len
   // keeps the multiplier within the accumulator bound
@@ -349,6 +524,13 @@ hipError_t launch_mul(OutOperand p, Operand a, Operand b, uint32_t count, hipStr
   if (a.len <= 512) return launch_mul_lanes<16, 38, 27>(p, a, b, count, stream);  // 608 digits >= 16384 bits
   hipLaunchKernelGGL(mul_general_kernel, dim3(div_round_up(count, 256)), dim3(256), 0, stream, p, a, b, count);
   return hipGetLastError();
+}
+
+hipError_t launch_sqr(OutOperand p, Operand a, uint32_t count, hipStream_t stream) {
+  // symmetric product scanning: each product of two different digits once
+  if (a.len > 16 && a.len <= 32) return launch_mul_cols<37, 37, true>(p, a, a, count, stream);
+  if (a.len > 48 && a.len <= 64) return launch_mul_cols<74, 74, true>(p, a, a, count, stream);
+  return launch_mul(p, a, a, count, stream);
 }
 
 }  // namespace xmp

@@ -37,6 +37,8 @@ hipError_t launch_copy_words(uint32_t *dst, const uint32_t *src, size_t words, h
 hipError_t launch_add(OutOperand s, Operand a, Operand b, uint32_t count, hipStream_t stream);
 hipError_t launch_sub(OutOperand d, Operand a, Operand b, uint32_t count, hipStream_t stream);
 hipError_t launch_mul(OutOperand p, Operand a, Operand b, uint32_t count, hipStream_t stream);
+// p = a^2 (truncated to p.len limbs): the symmetric kernels where a fits them
+hipError_t launch_sqr(OutOperand p, Operand a, uint32_t count, hipStream_t stream);
 
 // ---- division (k_div.hip) ----
 // q and/or r may have data == nullptr to skip that output.

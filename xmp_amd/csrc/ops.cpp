@@ -122,7 +122,7 @@ xmpError_t XMPAPI xmpIntegersSqrAsync(xmpHandle_t handle, xmpIntegers_t p, const
       return xmpErrorCuda;
   }
   Operand A = make_in(a, pol, 1);
-  if (launch_mul(make_out(p, buffer, pol, 0), A, A, count, handle->stream) != hipSuccess) return xmpErrorCuda;
+  if (launch_sqr(make_out(p, buffer, pol, 0), A, count, handle->stream) != hipSuccess) return xmpErrorCuda;
   return finish(handle, inplace, p, buffer);
 }
 xmpError_t XMPAPI xmpIntegersMulAsync(xmpHandle_t handle, xmpIntegers_t p, const xmpIntegers_t a,

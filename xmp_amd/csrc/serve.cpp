@@ -57,9 +57,9 @@ struct Request {
   Clock::time_point arrival;
 };
 
+// a completed request; its result words are in the server's result queue
 struct Done {
   uint64_t tag;
-  std::vector<uint32_t> result;
   double latency_us;
 };
 
@@ -88,14 +88,18 @@ struct _xmpServer_t {
     uint32_t r2_stride;
   };
   std::vector<Layout> lay_n, lay_pq;
-  size_t bytes_per_request = 0;
-  uint32_t max_inflight = 0;
+  size_t bytes_per_request = 0, budget = 0;
+  uint32_t cap = 0, max_inflight = 0;
 
   struct Slot {
-    hipStream_t stream = nullptr;     // staging and the kind-0 (single powm) work
-    hipStream_t stream_b = nullptr;   // the kind-1 (CRT) work, concurrent with kind 0
+    hipStream_t stream = nullptr;     // the kind-0 (single powm) exponentiation
+    hipStream_t stream_b = nullptr;   // the kind-1 (CRT) exponentiation, concurrent with kind 0
+    // the short kernels (index staging, imports, the CRT chain's divisions,
+    // multiply and add, exports): the same two streams, or streams confined
+    // to the reserved CUs so that they never wait behind exponentiations
+    hipStream_t short_a = nullptr, short_b = nullptr;
     hipEvent_t done = nullptr, fork = nullptr, join = nullptr;
-    uint32_t cap = 0;
+    hipEvent_t in_a = nullptr, pow_a = nullptr, in_b = nullptr, pow_b = nullptr;
     // pinned staging
     uint32_t *h_in = nullptr;     // [A base | A exp | B c] compact, cap*n32 each
     uint32_t *h_idx = nullptr;    // index arrays
@@ -109,19 +113,25 @@ struct _xmpServer_t {
     uint32_t *div_ws = nullptr;   // division workspace for divisors above the register kernels (else null)
     std::vector<Request> reqs;
     uint32_t na = 0, nb = 0;
-    bool busy = false;
   };
   std::vector<Slot> slots;
 
+  // Two host threads: the scheduler admits arrived requests into free slots
+  // and launches them; the completer retires finished slots and publishes
+  // their results.  Everything below is guarded by mu.
   std::mutex mu;
-  std::condition_variable cv;
+  std::condition_variable cv_sched, cv_drain;
   std::deque<Request> pending;
-  std::deque<Done> done;
-  uint64_t submitted = 0, completed = 0;
-  uint32_t inflight = 0;
-  std::atomic<bool> running{false};
-  std::thread worker;
+  std::deque<uint32_t> free_slots;      // slots the scheduler may fill
+  std::vector<uint32_t> launched;       // slots on the device, for the completer
+  std::deque<Done> done;                // completed requests in completion order ...
+  std::vector<uint32_t> done_words;     // ... and their results (n32 words each from done_head)
+  size_t done_head = 0;
+  uint64_t submitted = 0, completed = 0, batches = 0;
+  uint32_t inflight = 0, peak_inflight = 0, peak_batch = 0;
+  bool stopping = false, scheduler_exited = false;
   xmpError_t worker_error = xmpErrorSuccess;
+  std::thread scheduler, completer;
   // failure-handling test hook: with XMP_SERVER_FAIL_BATCH=k set at creation,
   // the k-th batch launch reports an error after enqueuing its kind-0 work
   uint32_t fail_batch = 0, launches = 0;
@@ -164,16 +174,23 @@ const _xmpServer_t::Layout &pick_layout(const std::vector<_xmpServer_t::Layout> 
   return set[best];
 }
 
+// `later` waits for what `earlier` has enqueued so far (nothing when they are one stream)
+hipError_t order(hipStream_t earlier, hipStream_t later, hipEvent_t ev) {
+  if (earlier == later) return hipSuccess;
+  hipError_t e = hipEventRecord(ev, earlier);
+  return e != hipSuccess ? e : hipStreamWaitEvent(later, ev, 0);
+}
+
 // Launch one batch on its slot's streams: the index staging, then the two
-// request kinds concurrently (kind 0 on the slot's stream, kind 1 on its
-// second stream, each importing its requests from the pinned input buffer and
-// exporting its results into the pinned output buffer), then the completion
-// event once both have finished.
+// request kinds concurrently, each importing its requests from the pinned
+// input buffer and exporting its results into the pinned output buffer, then
+// the completion event once both have finished.  The exponentiations run on
+// the slot's two main streams and everything else on its short streams.
 hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
   XMP_TRACE("xmpServer batch");
-  const uint32_t n32 = s->n32, h32 = s->h32;
-  const uint32_t na = sl.na, nb = sl.nb, cap = sl.cap;
-  hipStream_t st = sl.stream;
+  const uint32_t n32 = s->n32;
+  const uint32_t na = sl.na, nb = sl.nb, cap = s->cap;
+  hipStream_t sa = sl.short_a, sb = sl.short_b;
   // index layout in h_idx: keyA[cap] | keyPQ[2cap] | cidx[2cap] | lo[cap] | hi[cap] | keyB[cap]
   uint32_t *keyA = sl.h_idx, *keyPQ = keyA + cap, *cidx = keyPQ + 2 * cap, *lo = cidx + 2 * cap, *hi = lo + cap,
            *keyB = hi + cap;
@@ -194,10 +211,9 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
   // the index arrays are re-read by the kernels, so they are staged in device
   // memory (a one-limb import is a word copy)
   if ((e = launch_import(sl.d_idx, kIdxWords * cap, 1, (const uint8_t *)sl.g_idx, kIdxWords * cap, 1, -1, -1, 0, 4,
-                         nullptr, 0, st)) != hipSuccess)
+                         nullptr, 0, sa)) != hipSuccess)
     return e;
-  hipStream_t sb = sl.stream_b;
-  if ((e = hipEventRecord(sl.fork, st)) || (e = hipStreamWaitEvent(sb, sl.fork, 0))) return e;
+  if ((e = hipEventRecord(sl.fork, sa)) || (e = hipStreamWaitEvent(sb, sl.fork, 0))) return e;
   const bool inject_failure = s->fail_batch != 0 && ++s->launches == s->fail_batch;
   const size_t words = (size_t)cap * n32;
   // the imports read the rows this batch uses straight from the pinned buffer:
@@ -205,8 +221,9 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
   const uint8_t *in_base = (const uint8_t *)sl.g_in, *in_exp = (const uint8_t *)(sl.g_in + words),
                 *in_c = (const uint8_t *)(sl.g_in + 2 * words);
   if (na) {
-    if ((e = launch_import(sl.a_base.data, sl.a_base.stride, n32, in_base, na, n32, -1, -1, 0, 4, nullptr, 0, st)) ||
-        (e = launch_import(sl.a_exp.data, sl.a_exp.stride, n32, in_exp, na, n32, -1, -1, 0, 4, nullptr, 0, st)))
+    if ((e = launch_import(sl.a_base.data, sl.a_base.stride, n32, in_base, na, n32, -1, -1, 0, 4, nullptr, 0, sa)) ||
+        (e = launch_import(sl.a_exp.data, sl.a_exp.stride, n32, in_exp, na, n32, -1, -1, 0, 4, nullptr, 0, sa)) ||
+        (e = order(sa, sl.stream, sl.in_a)))
       return e;
     PowmArgs a;
     a.out = outop(sl.a_out);
@@ -220,9 +237,10 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
     a.start = 0;
     a.count = na;
     a.window_bits = ly.window;
-    if ((e = launch_powm_variant(ly.variant, a, st)) != hipSuccess) return e;
+    if ((e = launch_powm_variant(ly.variant, a, sl.stream)) != hipSuccess || (e = order(sl.stream, sa, sl.pow_a)))
+      return e;
     if ((e = launch_export((uint8_t *)sl.g_out, sl.a_out.data, sl.a_out.stride, n32, na, n32, -1, -1, 0, 4, nullptr,
-                           0, st)) != hipSuccess)
+                           0, sa)) != hipSuccess)
       return e;
   }
   if (inject_failure) return hipErrorLaunchFailure;
@@ -231,7 +249,8 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
     // c mod p_k and c mod q_k in one launch over 2*nb instances
     OutOperand none{nullptr, 0, 0, nullptr};
     if ((e = launch_divmod_ws(none, outop(sl.b_cm), op(sl.b_c, d_cidx, 2 * nb), op(s->PQ, d_keyPQ, 2 * nb), 2 * nb,
-                              sl.div_ws, sb)) != hipSuccess)
+                              sl.div_ws, sb)) != hipSuccess ||
+        (e = order(sb, sl.stream_b, sl.in_b)))
       return e;
     // mp = cm^dp mod p, mq = cm^dq mod q in one launch
     PowmArgs a;
@@ -246,7 +265,9 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
     a.start = 0;
     a.count = 2 * nb;
     a.window_bits = ly.window;
-    if ((e = launch_powm_variant(ly.variant, a, sb)) != hipSuccess) return e;
+    if ((e = launch_powm_variant(ly.variant, a, sl.stream_b)) != hipSuccess ||
+        (e = order(sl.stream_b, sb, sl.pow_b)))
+      return e;
     // sp = mp*cp, sq = mq*cq; r = (sp + sq) mod n
     if ((e = launch_mul(outop(sl.b_prod), op(s->CPQ, d_keyPQ, 2 * nb), op(sl.b_pow), 2 * nb, sb)) != hipSuccess)
       return e;
@@ -259,117 +280,132 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
                            nullptr, 0, sb)) != hipSuccess)
       return e;
   }
-  if ((e = hipEventRecord(sl.join, sb)) || (e = hipStreamWaitEvent(st, sl.join, 0))) return e;
-  (void)h32;
-  return hipEventRecord(sl.done, st);
+  if ((e = hipEventRecord(sl.join, sb)) || (e = hipStreamWaitEvent(sa, sl.join, 0))) return e;
+  return hipEventRecord(sl.done, sa);
 }
 
-// A batch whose launch or execution failed: wait until nothing of it is still
-// queued on the slot's streams, then count its requests as completed without
-// publishing results, and make the error sticky (Poll, Drain and RunPoisson
-// return it; no further batch is admitted).
-void retire_failed(xmpServer_t s, _xmpServer_t::Slot &sl) {
-  (void)hipStreamSynchronize(sl.stream);
-  (void)hipStreamSynchronize(sl.stream_b);
+// Wait until nothing of a slot's batch is still queued on any of its streams.
+void sync_slot(_xmpServer_t::Slot &sl) {
+  for (hipStream_t st : {sl.stream, sl.stream_b, sl.short_a, sl.short_b})
+    if (st) (void)hipStreamSynchronize(st);
   (void)hipGetLastError();
-  {
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->worker_error = xmpErrorCuda;
-    s->completed += sl.reqs.size();
-    s->inflight -= (uint32_t)sl.reqs.size();
-  }
-  s->cv.notify_all();
-  sl.reqs.clear();
-  sl.busy = false;
 }
 
-void worker_loop(xmpServer_t s) {
+// A batch whose launch or execution failed (the slot's streams are idle):
+// count its requests as completed without publishing results, make the error
+// sticky (Poll, Drain and RunPoisson return it; no further batch is
+// admitted) and free the slot.  Called with mu held.
+void fail_slot_locked(xmpServer_t s, uint32_t slot) {
+  _xmpServer_t::Slot &sl = s->slots[slot];
+  s->worker_error = xmpErrorCuda;
+  s->completed += sl.reqs.size();
+  s->inflight -= (uint32_t)sl.reqs.size();
+  sl.reqs.clear();
+  s->free_slots.push_back(slot);
+}
+
+// Scheduler thread: whenever requests are pending, a slot is free and the
+// budget allows, move up to a slot's capacity of them into it and launch.
+void scheduler_loop(xmpServer_t s) {
   (void)hipSetDevice(s->h->device);
   roctxNameOsThread("xmpServer scheduler");
   const uint32_t n32 = s->n32;
+  std::unique_lock<std::mutex> lk(s->mu);
   while (true) {
-    bool progressed = false;
-    // 1. retire finished batches
-    for (auto &sl : s->slots) {
-      if (!sl.busy) continue;
-      hipError_t q = hipEventQuery(sl.done);
-      if (q == hipErrorNotReady) continue;
-      if (q != hipSuccess) {
-        // the batch failed on the device: its results are not published
-        retire_failed(s, sl);
-        progressed = true;
-        continue;
-      }
-      auto now = Clock::now();
-      std::vector<Done> out;
-      out.reserve(sl.reqs.size());
-      const size_t words = (size_t)sl.cap * n32;
-      for (uint32_t i = 0; i < sl.reqs.size(); i++) {
-        const Request &r = sl.reqs[i];
-        const uint32_t *src = (i < sl.na) ? sl.h_out + (size_t)i * n32 : sl.h_out + words + (size_t)(i - sl.na) * n32;
-        Done d;
-        d.tag = r.tag;
-        d.result.assign(src, src + n32);
-        d.latency_us = std::chrono::duration<double, std::micro>(now - r.arrival).count();
-        out.push_back(std::move(d));
-      }
-      {
-        std::lock_guard<std::mutex> lk(s->mu);
-        for (auto &d : out) s->done.push_back(std::move(d));
-        s->completed += sl.reqs.size();
-        s->inflight -= (uint32_t)sl.reqs.size();
-      }
-      s->cv.notify_all();
-      sl.reqs.clear();
-      sl.busy = false;
-      progressed = true;
+    auto ready = [&] {
+      return !s->pending.empty() && !s->free_slots.empty() && s->inflight < s->max_inflight;
+    };
+    s->cv_sched.wait(lk, [&] {
+      return s->worker_error != xmpErrorSuccess || ready() || (s->stopping && s->pending.empty());
+    });
+    if (s->worker_error != xmpErrorSuccess || !ready()) break;   // an error, or stopping with nothing pending
+    const uint32_t slot = s->free_slots.front();
+    s->free_slots.pop_front();
+    _xmpServer_t::Slot &sl = s->slots[slot];
+    const uint32_t take = std::min<uint32_t>({s->cap, s->max_inflight - s->inflight, (uint32_t)s->pending.size()});
+    // kind-0 requests first, then kind 1 (slot layout); order within a kind is arrival order
+    std::vector<Request> b;
+    sl.reqs.clear();
+    for (uint32_t i = 0; i < take; i++) {
+      Request r = std::move(s->pending.front());
+      s->pending.pop_front();
+      (r.kind == 0 ? sl.reqs : b).push_back(std::move(r));
     }
-    // 2. admit pending requests into a free slot, within the HBM budget
-    // (nothing more is admitted once a batch has failed: the error is sticky)
-    for (auto &sl : s->slots) {
-      if (sl.busy) continue;
-      {
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (s->pending.empty() || s->worker_error != xmpErrorSuccess) break;
-        uint32_t budget = s->max_inflight > s->inflight ? s->max_inflight - s->inflight : 0;
-        uint32_t take = std::min<uint32_t>({sl.cap, budget, (uint32_t)s->pending.size()});
-        if (take == 0) break;
-        // kind-0 requests first, then kind 1 (slot layout); order within a kind is arrival order
-        std::vector<Request> a, b;
-        a.reserve(take);
-        b.reserve(take);
-        for (uint32_t i = 0; i < take; i++) {
-          Request r = std::move(s->pending.front());
-          s->pending.pop_front();
-          (r.kind == 0 ? a : b).push_back(std::move(r));
-        }
-        sl.na = (uint32_t)a.size();
-        sl.nb = (uint32_t)b.size();
-        sl.reqs = std::move(a);
-        for (auto &r : b) sl.reqs.push_back(std::move(r));
-        s->inflight += take;
-      }
-      const size_t words = (size_t)sl.cap * n32;
-      for (uint32_t i = 0; i < sl.na; i++) {
-        std::memcpy(sl.h_in + (size_t)i * n32, sl.reqs[i].base.data(), n32 * 4);
-        std::memcpy(sl.h_in + words + (size_t)i * n32, sl.reqs[i].exp.data(), n32 * 4);
-      }
-      for (uint32_t j = 0; j < sl.nb; j++)
-        std::memcpy(sl.h_in + 2 * words + (size_t)j * n32, sl.reqs[sl.na + j].base.data(), n32 * 4);
-      progressed = true;
-      if (launch_batch(s, sl) != hipSuccess) {
-        // part of the batch may be enqueued on either stream: wait for both
-        // before the slot's buffers are reused, and fail its requests
-        retire_failed(s, sl);
-        continue;
-      }
-      sl.busy = true;
+    sl.na = (uint32_t)sl.reqs.size();
+    sl.nb = (uint32_t)b.size();
+    for (auto &r : b) sl.reqs.push_back(std::move(r));
+    s->inflight += take;
+    s->peak_inflight = std::max(s->peak_inflight, s->inflight);
+    s->peak_batch = std::max(s->peak_batch, take);
+    s->batches++;
+    lk.unlock();
+    const size_t words = (size_t)s->cap * n32;
+    for (uint32_t i = 0; i < sl.na; i++) {
+      std::memcpy(sl.h_in + (size_t)i * n32, sl.reqs[i].base.data(), n32 * 4);
+      std::memcpy(sl.h_in + words + (size_t)i * n32, sl.reqs[i].exp.data(), n32 * 4);
     }
-    if (!s->running.load()) {
-      bool any_busy = false;
-      for (auto &sl : s->slots) any_busy |= sl.busy;
+    for (uint32_t j = 0; j < sl.nb; j++)
+      std::memcpy(sl.h_in + 2 * words + (size_t)j * n32, sl.reqs[sl.na + j].base.data(), n32 * 4);
+    const bool ok = launch_batch(s, sl) == hipSuccess;
+    // part of a failed batch may be enqueued: wait for all of it before the
+    // slot's buffers can be reused
+    if (!ok) sync_slot(sl);
+    lk.lock();
+    if (ok) {
+      s->launched.push_back(slot);
+    } else {
+      fail_slot_locked(s, slot);
+      s->cv_drain.notify_all();
+    }
+  }
+  s->scheduler_exited = true;
+}
+
+// Completer thread: polls the launched slots' completion events and
+// publishes every finished slot's results (in whatever order slots finish).
+void completer_loop(xmpServer_t s) {
+  (void)hipSetDevice(s->h->device);
+  roctxNameOsThread("xmpServer completer");
+  const uint32_t n32 = s->n32;
+  std::vector<uint32_t> slots;
+  while (true) {
+    {
       std::lock_guard<std::mutex> lk(s->mu);
-      if (!any_busy && (s->pending.empty() || s->worker_error != xmpErrorSuccess)) break;
+      if (s->launched.empty() && s->scheduler_exited) break;
+      slots = s->launched;
+    }
+    bool progressed = false;
+    for (uint32_t slot : slots) {
+      _xmpServer_t::Slot &sl = s->slots[slot];
+      const hipError_t q = hipEventQuery(sl.done);
+      if (q == hipErrorNotReady) continue;
+      progressed = true;
+      const auto now = Clock::now();
+      if (q != hipSuccess) sync_slot(sl);
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->launched.erase(std::find(s->launched.begin(), s->launched.end(), slot));
+        if (q != hipSuccess) {
+          // the batch failed on the device: its results are not published
+          fail_slot_locked(s, slot);
+        } else {
+          const size_t words = (size_t)s->cap * n32;
+          if (s->done_head > 0 && s->done_head * 2 >= s->done_words.size()) {
+            s->done_words.erase(s->done_words.begin(), s->done_words.begin() + (ptrdiff_t)s->done_head);
+            s->done_head = 0;
+          }
+          s->done_words.insert(s->done_words.end(), sl.h_out, sl.h_out + (size_t)sl.na * n32);
+          s->done_words.insert(s->done_words.end(), sl.h_out + words, sl.h_out + words + (size_t)sl.nb * n32);
+          for (const Request &r : sl.reqs)
+            s->done.push_back({r.tag, std::chrono::duration<double, std::micro>(now - r.arrival).count()});
+          s->completed += sl.reqs.size();
+          s->inflight -= (uint32_t)sl.reqs.size();
+          sl.reqs.clear();
+          s->free_slots.push_back(slot);
+        }
+      }
+      s->cv_sched.notify_one();
+      s->cv_drain.notify_all();
     }
     if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
   }
@@ -378,8 +414,7 @@ void worker_loop(xmpServer_t s) {
 void free_server(xmpServer_t s) {
   xmpHandle_t h = s->h;
   for (auto &sl : s->slots) {
-    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-    if (sl.stream_b) (void)hipStreamSynchronize(sl.stream_b);
+    sync_slot(sl);
     for (DevArray *a : {&sl.a_base, &sl.a_exp, &sl.a_out, &sl.b_c, &sl.b_cm, &sl.b_pow, &sl.b_prod, &sl.b_sum,
                         &sl.b_out})
       if (a->data) h->df(a->data);
@@ -390,8 +425,10 @@ void free_server(xmpServer_t s) {
     if (sl.h_in) (void)hipHostFree(sl.h_in);
     if (sl.h_idx) (void)hipHostFree(sl.h_idx);
     if (sl.h_out) (void)hipHostFree(sl.h_out);
-    for (hipEvent_t ev : {sl.done, sl.fork, sl.join})
+    for (hipEvent_t ev : {sl.done, sl.fork, sl.join, sl.in_a, sl.pow_a, sl.in_b, sl.pow_b})
       if (ev) (void)hipEventDestroy(ev);
+    if (sl.short_a && sl.short_a != sl.stream) (void)hipStreamDestroy(sl.short_a);
+    if (sl.short_b && sl.short_b != sl.stream_b) (void)hipStreamDestroy(sl.short_b);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
     if (sl.stream_b) (void)hipStreamDestroy(sl.stream_b);
   }
@@ -453,49 +490,93 @@ xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const
       (e = xmpIntegersImport(handle, s->DPQ, s->h32, -1, 4, 0, 0, dpq.data(), 2 * nkeys)) ||
       (e = xmpIntegersImport(handle, s->CPQ, s->n32, -1, 4, 0, 0, cpq.data(), 2 * nkeys)))
     return fail(e);
-  const uint32_t cap = config->max_batch;
-  // candidate layouts: the tuner's choice at batch sizes 64, 256, ... up to the capacity
-  size_t tab_a = 0, tab_b = 0, req_a = 0, req_b = 0;
-  for (int kind = 0; kind < 2; kind++) {
-    auto &set = kind == 0 ? s->lay_n : s->lay_pq;
-    const uint32_t pbits = kind == 0 ? bits : bits / 2, mult = kind == 0 ? 1 : 2;
-    xmpIntegers_t mods = kind == 0 ? s->N : s->PQ;
-    std::vector<uint32_t> counts;
-    for (uint32_t c = 64; c < cap; c *= 4) counts.push_back(c * mult);
-    counts.push_back(cap * mult);
-    for (uint32_t c : counts) {
-      PowmChoice ch;
-      if ((e = choose_powm(handle, pbits, pbits, c, xmpAlgorithmDefault, &ch))) return fail(e);
-      _xmpServer_t::Layout ly{c, ch.variant, ch.window_bits, nullptr, 0};
-      for (auto &o : set)
-        if (o.variant == ch.variant) {
-          ly.r2 = o.r2;
-          ly.r2_stride = o.r2_stride;
-        }
-      if (!ly.r2 && (e = montgomery_prepare(handle, mods, ch.variant, &ly.r2, &ly.r2_stride)))
-        return fail(e);
-      set.push_back(ly);
-      const PowmVariant v = powm_variant(ch.variant);
-      size_t &tab = kind == 0 ? tab_a : tab_b;
-      size_t &req = kind == 0 ? req_a : req_b;
-      tab = std::max(tab, powm_table_words(v, round_up(cap * mult, kStrideAlign), ch.window_bits));
-      req = std::max(req, powm_table_words(v, mult, ch.window_bits) * 4);
+  // Layouts for batch sizes 64, 256, ... up to the capacity; with prepare,
+  // R^2 mod m for each distinct layout; returns the table words one slot
+  // needs per kind and the device bytes one in-flight request holds.
+  size_t tab_a = 0, tab_b = 0;
+  auto plan = [&](uint32_t cap, bool prepare, size_t *per_request) -> xmpError_t {
+    size_t req_a = 0, req_b = 0;
+    tab_a = tab_b = 0;
+    for (int kind = 0; kind < 2; kind++) {
+      auto &set = kind == 0 ? s->lay_n : s->lay_pq;
+      const uint32_t pbits = kind == 0 ? bits : bits / 2, mult = kind == 0 ? 1 : 2;
+      xmpIntegers_t mods = kind == 0 ? s->N : s->PQ;
+      std::vector<uint32_t> counts;
+      for (uint32_t c = 64; c < cap; c *= 4) counts.push_back(c * mult);
+      counts.push_back(cap * mult);
+      for (uint32_t c : counts) {
+        PowmChoice ch;
+        xmpError_t err = choose_powm(handle, pbits, pbits, c, xmpAlgorithmDefault, &ch);
+        if (err != xmpErrorSuccess) return err;
+        const PowmVariant v = powm_variant(ch.variant);
+        size_t &tab = kind == 0 ? tab_a : tab_b;
+        size_t &req = kind == 0 ? req_a : req_b;
+        tab = std::max(tab, powm_table_words(v, round_up(cap * mult, kStrideAlign), ch.window_bits));
+        req = std::max(req, powm_table_words(v, mult, ch.window_bits) * 4);
+        if (!prepare) continue;
+        _xmpServer_t::Layout ly{c, ch.variant, ch.window_bits, nullptr, 0};
+        for (auto &o : set)
+          if (o.variant == ch.variant) {
+            ly.r2 = o.r2;
+            ly.r2_stride = o.r2_stride;
+          }
+        if (!ly.r2 && (err = montgomery_prepare(handle, mods, ch.variant, &ly.r2, &ly.r2_stride))) return err;
+        set.push_back(ly);
+      }
     }
+    // staging, intermediates and the table share of one request (kind-1 worst case)
+    *per_request = 4 * ((size_t)s->n32 * 8 + (size_t)s->h32 * 4 + 2 * ((size_t)s->n32 + s->h32) + s->n32 + 2 +
+                        kIdxWords) +
+                   std::max(req_a, req_b);
+    return xmpErrorSuccess;
+  };
+  // The HBM budget: the configured bytes, else half of the device memory free
+  // now.  Every slot's buffers are allocated here for its capacity, so the
+  // capacity shrinks until all slots fit the budget, and the scheduler never
+  // has more requests in flight than the budget holds.
+  size_t budget = config->hbm_budget;
+  if (!budget) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return fail(xmpErrorCuda);
+    budget = free_b / 2;
   }
-  // device bytes one in-flight request holds (kind-1 worst case): staging, intermediates, table share
-  s->bytes_per_request = 4 * ((size_t)s->n32 * 8 + (size_t)s->h32 * 4 + 2 * ((size_t)s->n32 + s->h32) + s->n32 + 2) +
-                         std::max(req_a, req_b);
-  size_t budget = config->hbm_budget ? config->hbm_budget : s->bytes_per_request * cap * config->streams;
-  s->max_inflight = (uint32_t)std::max<size_t>(1, budget / s->bytes_per_request);
+  size_t bpr = 0;
+  if ((e = plan(config->max_batch, false, &bpr))) return fail(e);
+  const size_t fit = budget / (bpr * config->streams);
+  s->cap = (uint32_t)std::max<size_t>(1, std::min<size_t>(config->max_batch, fit));
+  if ((e = plan(s->cap, true, &s->bytes_per_request))) return fail(e);
+  s->budget = budget;
+  s->max_inflight = (uint32_t)std::max<size_t>(1, std::min<size_t>(budget / s->bytes_per_request,
+                                                                     (size_t)s->cap * config->streams));
+  const uint32_t cap = s->cap;
+  // CU masks: the exponentiations avoid the reserved CUs, the short kernels use only them
+  std::vector<uint32_t> mask_main, mask_short;
+  if (config->reserved_cus) {
+    const uint32_t cus = handle->cuCount;
+    if (config->reserved_cus >= cus) return fail(xmpErrorInvalidParameter);
+    mask_main.assign((cus + 31) / 32, 0);
+    mask_short.assign((cus + 31) / 32, 0);
+    for (uint32_t c = 0; c < cus; c++) (c < config->reserved_cus ? mask_short : mask_main)[c / 32] |= 1u << (c % 32);
+  }
+  auto make_stream = [&](hipStream_t *st, const std::vector<uint32_t> &mask) {
+    return mask.empty() ? hipStreamCreate(st) : hipExtStreamCreateWithCUMask(st, (uint32_t)mask.size(), mask.data());
+  };
   s->slots.resize(config->streams);
-  for (auto &sl : s->slots) {
-    sl.cap = cap;
+  for (uint32_t k = 0; k < config->streams; k++) {
+    auto &sl = s->slots[k];
     const size_t words = (size_t)cap * s->n32;
-    if (hipStreamCreate(&sl.stream) != hipSuccess || hipStreamCreate(&sl.stream_b) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.join, hipEventDisableTiming) != hipSuccess ||
-        hipHostMalloc((void **)&sl.h_in, sizeof(uint32_t) * 3 * words, kPinned) != hipSuccess ||
+    if (make_stream(&sl.stream, mask_main) != hipSuccess || make_stream(&sl.stream_b, mask_main) != hipSuccess)
+      return fail(xmpErrorCuda);
+    if (config->reserved_cus) {
+      if (make_stream(&sl.short_a, mask_short) != hipSuccess || make_stream(&sl.short_b, mask_short) != hipSuccess)
+        return fail(xmpErrorCuda);
+    } else {
+      sl.short_a = sl.stream;
+      sl.short_b = sl.stream_b;
+    }
+    for (hipEvent_t *ev : {&sl.done, &sl.fork, &sl.join, &sl.in_a, &sl.pow_a, &sl.in_b, &sl.pow_b})
+      if (hipEventCreateWithFlags(ev, hipEventDisableTiming) != hipSuccess) return fail(xmpErrorCuda);
+    if (hipHostMalloc((void **)&sl.h_in, sizeof(uint32_t) * 3 * words, kPinned) != hipSuccess ||
         hipHostMalloc((void **)&sl.h_idx, sizeof(uint32_t) * kIdxWords * cap, kPinned) != hipSuccess ||
         hipHostMalloc((void **)&sl.h_out, sizeof(uint32_t) * 2 * words, kPinned) != hipSuccess ||
         hipHostGetDevicePointer((void **)&sl.g_in, sl.h_in, 0) != hipSuccess ||
@@ -517,10 +598,11 @@ xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const
         alloc_array(handle, sl.b_prod, s->n32 + s->h32, 2 * cap) ||
         alloc_array(handle, sl.b_sum, s->n32 + s->h32 + 1, cap) || alloc_array(handle, sl.b_out, s->n32, cap))
       return fail(xmpErrorInvalidCudaMalloc);
+    s->free_slots.push_back(k);
   }
   if (hipDeviceSynchronize() != hipSuccess) return fail(xmpErrorCuda);
-  s->running = true;
-  s->worker = std::thread(worker_loop, s);
+  s->scheduler = std::thread(scheduler_loop, s);
+  s->completer = std::thread(completer_loop, s);
   *server = s;
   return xmpErrorSuccess;
 }
@@ -541,6 +623,7 @@ xmpError_t XMPAPI xmpServerSubmit(xmpServer_t server, uint32_t kind, uint32_t ke
     server->pending.push_back(std::move(r));
     server->submitted++;
   }
+  server->cv_sched.notify_one();
   return xmpErrorSuccess;
 }
 
@@ -548,15 +631,22 @@ xmpError_t XMPAPI xmpServerPoll(xmpServer_t server, uint32_t max, uint64_t *tags
                                 double *latency_us, uint32_t *n_out) {
   XMP_CHECK_NE(server, nullptr);
   XMP_CHECK_NE(n_out, nullptr);
+  const uint32_t n32 = server->n32;
   std::lock_guard<std::mutex> lk(server->mu);
   uint32_t k = 0;
   while (k < max && !server->done.empty()) {
-    Done &d = server->done.front();
+    const Done &d = server->done.front();
     if (tags) tags[k] = d.tag;
-    if (results) std::memcpy(results + (size_t)k * server->n32, d.result.data(), sizeof(uint32_t) * server->n32);
+    if (results)
+      std::memcpy(results + (size_t)k * n32, server->done_words.data() + server->done_head, sizeof(uint32_t) * n32);
     if (latency_us) latency_us[k] = d.latency_us;
     server->done.pop_front();
+    server->done_head += n32;
     k++;
+  }
+  if (server->done.empty()) {
+    server->done_words.clear();
+    server->done_head = 0;
   }
   *n_out = k;
   return server->worker_error;
@@ -565,14 +655,33 @@ xmpError_t XMPAPI xmpServerPoll(xmpServer_t server, uint32_t max, uint64_t *tags
 xmpError_t XMPAPI xmpServerDrain(xmpServer_t server) {
   XMP_CHECK_NE(server, nullptr);
   std::unique_lock<std::mutex> lk(server->mu);
-  server->cv.wait(lk, [&] { return server->completed == server->submitted || server->worker_error; });
+  server->cv_drain.wait(lk, [&] { return server->completed == server->submitted || server->worker_error; });
   return server->worker_error;
+}
+
+xmpError_t XMPAPI xmpServerGetStats(xmpServer_t server, xmpServerStats_t *stats) {
+  XMP_CHECK_NE(server, nullptr);
+  XMP_CHECK_NE(stats, nullptr);
+  std::lock_guard<std::mutex> lk(server->mu);
+  stats->batches = server->batches;
+  stats->slot_capacity = server->cap;
+  stats->max_inflight = server->max_inflight;
+  stats->peak_inflight = server->peak_inflight;
+  stats->peak_batch = server->peak_batch;
+  stats->bytes_per_request = server->bytes_per_request;
+  stats->hbm_budget = server->budget;
+  return xmpErrorSuccess;
 }
 
 xmpError_t XMPAPI xmpServerDestroy(xmpServer_t server) {
   XMP_CHECK_NE(server, nullptr);
-  server->running = false;
-  if (server->worker.joinable()) server->worker.join();
+  {
+    std::lock_guard<std::mutex> lk(server->mu);
+    server->stopping = true;
+  }
+  server->cv_sched.notify_all();
+  if (server->scheduler.joinable()) server->scheduler.join();
+  if (server->completer.joinable()) server->completer.join();
   XMP_SET_DEVICE(server->h);
   free_server(server);
   delete server;

@@ -125,7 +125,13 @@ def tuner_load(json_text):
 
 
 class ServerConfig(ctypes.Structure):
-    _fields_ = [("streams", _u32), ("max_batch", _u32), ("hbm_budget", ctypes.c_size_t)]
+    _fields_ = [("streams", _u32), ("max_batch", _u32), ("hbm_budget", ctypes.c_size_t), ("reserved_cus", _u32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_uint64), ("slot_capacity", _u32), ("max_inflight", _u32),
+                ("peak_inflight", _u32), ("peak_batch", _u32), ("bytes_per_request", ctypes.c_size_t),
+                ("hbm_budget", ctypes.c_size_t)]
 
 
 def declare_server(L):
@@ -138,6 +144,7 @@ def declare_server(L):
         "xmpServerSubmit": [_vp, _u32, _u32, _vp, _vp, ctypes.c_uint64],
         "xmpServerPoll": [_vp, _u32, _vp, _vp, _vp, u32p],
         "xmpServerDrain": [_vp],
+        "xmpServerGetStats": [_vp, ctypes.POINTER(ServerStats)],
         "xmpServerRunPoisson": [_vp, ctypes.c_double, _u32, ctypes.c_uint64, _vp, _vp, _vp, _vp, _vp, _vp,
                                 ctypes.POINTER(ctypes.c_double)],
     }
@@ -183,14 +190,14 @@ def make_keys(nkeys, bits, seed):
 class Server:
     """Continuous-batching powm server over resident keys (xmpServerCreate)."""
 
-    def __init__(self, handle, keys, bits, streams=4, max_batch=1024, hbm_budget=0):
+    def __init__(self, handle, keys, bits, streams=4, max_batch=1024, hbm_budget=0, reserved_cus=0):
         import numpy as np
         from . import check, lib
         L = lib()
         declare_server(L)
         self.handle, self.bits, self.n32 = handle, bits, bits // 32
         self.keys = {k: np.ascontiguousarray(v, np.uint32) for k, v in keys.items()}
-        cfg = ServerConfig(streams, max_batch, hbm_budget)
+        cfg = ServerConfig(streams, max_batch, hbm_budget, reserved_cus)
         self.s = _vp()
         kk = self.keys
         check(L.xmpServerCreate(handle.h, ctypes.byref(self.s), ctypes.byref(cfg), bits, kk["n"].shape[0],
@@ -219,6 +226,13 @@ class Server:
                                   ctypes.byref(n)), "xmpServerPoll")
         k = n.value
         return tags[:k], res[:k], lat[:k]
+
+    def stats(self):
+        """Admission counters (xmpServerGetStats) as a dict."""
+        from . import check, lib
+        st = ServerStats()
+        check(lib().xmpServerGetStats(self.s, ctypes.byref(st)), "xmpServerGetStats")
+        return {name: getattr(st, name) for name, _ in ServerStats._fields_}
 
     def run_poisson(self, rate, kinds, keys, bases, exps, seed=1):
         import numpy as np
