@@ -133,6 +133,21 @@ xmpError_t XMPAPI xmpTunerRun(xmpHandle_t handle, uint32_t precision, uint32_t e
 /* Load a tuning table (JSON text as written by xmpTunerRun or the tune tool). */
 xmpError_t XMPAPI xmpTunerLoad(const char *json);
 
+/* ---------------- allocation-free exponentiation ---------------- */
+/* No counterpart in the reference (whose xmpIntegersPowm grows the handle's
+ * scratch with cudaMalloc, src/xmp.cu:105-129).  For callers that cannot call
+ * hipMalloc: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation.  Same arguments, index slots, results and errors as
+ * xmpIntegersPowmAsync, but the call allocates nothing: its workspace (R^2
+ * mod m, the window table, an in-place output) is a 256 MiB device array
+ * inside the library, one per device, held from the first call on that
+ * device until the process ends.  A call whose smallest plan (window down to
+ * 2 bits, chunks down to 64 instances) does not fit 256 MiB returns
+ * xmpErrorIncreaseScratchLimit.  Calls on one device share that array:
+ * calls that overlap on one device race (order them on one stream). */
+xmpError_t XMPAPI xmpIntegersPowmNoAllocAsync(xmpHandle_t handle, xmpIntegers_t out, const xmpIntegers_t base,
+                                              const xmpIntegers_t exp, const xmpIntegers_t mod, uint32_t count);
+
 /* ---------------- diagnostics ---------------- */
 /* Issue-rate probe: independent v_mad_u64_u32 streams on every CU with
  * `waves_per_simd` waves per SIMD (1..8); returns multiply-adds per clock per

@@ -34,6 +34,7 @@ SOURCES = [
     "k_powm_latency.hip",
     "k_powm_wide.hip",
     "k_probe.hip",
+    "k_noalloc.hip",
 ]
 
 COMMON = [

@@ -21,6 +21,7 @@ def declare(L):
         "xmpBatchLaunchPlainAsync": [_vp, _vp],
         "xmpTunerRun": [_vp, _u32, _u32, _u32, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(_u32)],
         "xmpTunerLoad": [ctypes.c_char_p],
+        "xmpIntegersPowmNoAllocAsync": [_vp, _vp, _vp, _vp, _vp, _u32],
         "xmpProbeMadRate": [_vp, _u32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
     }
     for name, args in sig.items():
