@@ -139,10 +139,12 @@ def run_mixed(h, stream, total, seed, reps=2):
             "batched_over_plain": round(plain_ms / batched_ms, 3)}, samples
 
 
-SERVE_SLOTS, SERVE_BATCH = 4, 8192
+# 600,000 requests per rate point: the last batch's drain (20-30 ms) is then a
+# few per cent of the run instead of a tenth at 800 K req/s
+SERVE_SLOTS, SERVE_BATCH, SERVE_REQUESTS = 4, 8192, 600000
 
 
-def run_serving(h, nkeys, requests, seed, reserved_cus=0):
+def run_serving(h, nkeys, requests, seed, reserved_cus=0, slots=SERVE_SLOTS, batch=SERVE_BATCH, layout_scale=1):
     """Serving loop over `nkeys` resident 2048-bit keys.  Saturation is the
     throughput achieved under Poisson arrivals offered far above capacity
     (3 M req/s); then Poisson arrivals at fixed 400 K and 800 K req/s and at
@@ -156,9 +158,9 @@ def run_serving(h, nkeys, requests, seed, reserved_cus=0):
     nmat = keys["n"][key_idx]
     bases[:, -1] = (bases[:, -1].astype(np.uint64) % nmat[:, -1].astype(np.uint64)).astype(np.uint32)
     exps = inputs.uniform(seed + 2, requests, 64)
-    srv = Server(h, keys, 2048, streams=SERVE_SLOTS, max_batch=SERVE_BATCH, reserved_cus=reserved_cus)
+    srv = Server(h, keys, 2048, streams=slots, max_batch=batch, reserved_cus=reserved_cus, layout_scale=layout_scale)
     out = {"keys": nkeys, "requests": requests, "mix": "50% 2048-bit powm, 50% CRT pairs of 1024-bit moduli",
-           "streams": SERVE_SLOTS, "max_batch": SERVE_BATCH, "reserved_cus": reserved_cus}
+           "streams": slots, "max_batch": batch, "reserved_cus": reserved_cus, "layout_scale": layout_scale}
 
     def point(rate, seed_):
         res, lat, wall = srv.run_poisson(rate, kinds, key_idx, bases, exps, seed=seed_)
@@ -189,7 +191,8 @@ def run_only(args):
     h.set_stream(stream)
     lib = xmp_amd.lib()
     if args.only == "serve":
-        res, _ = run_serving(h, 1024, 200000, 99, reserved_cus=args.serve_reserved_cus)
+        res, _ = run_serving(h, 1024, SERVE_REQUESTS, 99, reserved_cus=args.serve_reserved_cus, slots=args.serve_slots,
+                             batch=args.serve_batch, layout_scale=args.serve_layout_scale)
         print(json.dumps(res), flush=True)
         h.close()
         return
@@ -248,6 +251,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-reserved-cus", type=int, default=0,
                     help="CUs the serving loop keeps for its short kernels (0 = none)")
+    ap.add_argument("--serve-slots", type=int, default=SERVE_SLOTS, help="serving batch slots")
+    ap.add_argument("--serve-batch", type=int, default=SERVE_BATCH, help="serving requests per slot")
+    ap.add_argument("--serve-layout-scale", type=int, default=1,
+                    help="serving layout chosen as for a batch this many times larger (0 = slots in flight)")
     ap.add_argument("--only", choices=["powm", "add", "mul", "sqr", "copy", "mixed", "serve"],
                     help="run just one kernel (for counter profiling): the 2048-bit powm kernel with resident "
                          "constants, the 2048-bit add, or a plain 256 MiB import copy for counter calibration")
@@ -445,7 +452,9 @@ def main():
         log("mixed batch: batched %.1f ms, plain %.1f ms (x%.2f)" % (
             mixed["batched_ms"], mixed["plain_ms"], mixed["batched_over_plain"]))
         # ---- serving: 1024 resident keys, Poisson arrivals
-        serving, serve_sample = run_serving(h, 1024, 200000, 99, reserved_cus=args.serve_reserved_cus)
+        serving, serve_sample = run_serving(h, 1024, SERVE_REQUESTS, 99, reserved_cus=args.serve_reserved_cus,
+                                            slots=args.serve_slots, batch=args.serve_batch,
+                                            layout_scale=args.serve_layout_scale)
         extra["serving"] = serving
         log("serving: %s" % json.dumps(serving))
 

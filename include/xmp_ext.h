@@ -81,6 +81,9 @@ typedef struct {
                              multiply and add, exports) on two more streams per slot, which the
                              exponentiations do not use; 0 = none (set GPU_MAX_HW_QUEUES >=
                              4 * streams when used) */
+  uint32_t layout_scale;  /* a batch's Montgomery layout is the tuner's choice for a batch this many
+                             times larger; 0 = the number of slots on the device when it launches
+                             (they share it), 1 = the batch alone */
 } xmpServerConfig_t;
 
 typedef struct {

@@ -283,3 +283,52 @@ def test_div_invariant_full_size(handle):
     wq, wr = gmp(O.GO_DIV, sample.size, 128, a[sample], b[sample], out2_limbs=64)
     assert_rows_equal(q[sample], wq, "q sample")
     assert_rows_equal(r[sample], wr, "r sample")
+
+
+def _addback_cases(nb, n, seed):
+    """Dividends and divisors whose quotient estimate from the top limbs is one
+    too large (Knuth D's add-back): V's limbs below its top two are all ones,
+    U = (q + 1) * (V's top two limbs, zeros below), so U // V == q."""
+    rng = np.random.default_rng(seed)
+    B = 1 << 32
+    a = np.zeros((n, nb + 1), np.uint32)
+    b = np.zeros((n, nb), np.uint32)
+    for i in range(n):
+        vt = int(rng.integers(0, B)) | 0x80000000
+        vt2 = int(rng.integers(0, B))
+        top = vt * B ** (nb - 1) + vt2 * B ** (nb - 2)
+        v = top + B ** (nb - 2) - 1
+        q = int(rng.integers(1, B - 1))
+        u = (q + 1) * top
+        assert u // v == q
+        a[i] = O.from_int(u, nb + 1)
+        b[i] = O.from_int(v, nb)
+    return a, b
+
+
+@pytest.mark.parametrize("nb", [32, 64])
+def test_div_addback_and_both_lane_layouts(handle, nb):
+    """The add-back path on every instance, through the two-lane kernel
+    (fewer than 131,072 divisions) and the one-lane kernel (more), and random
+    divisions at both counts against GMP."""
+    n = 256
+    a, b = _addback_cases(nb, n, 1000 + nb)
+    for reps in (1, 600):   # 256 and 153,600 divisions
+        aa, bb = np.tile(a, (reps, 1)), np.tile(b, (reps, 1))
+        A, B = handle.from_array(aa), handle.from_array(bb)
+        Q, R = handle.integers(32 * (nb + 1), n * reps), handle.integers(32 * nb, n * reps)
+        handle.divmod(Q, R, A, B)
+        wq, wr = gmp(O.GO_DIV, n, nb + 1, a, b, out2_limbs=nb)
+        assert_rows_equal(Q.get()[-n:], wq, "add-back q x%d" % reps)
+        assert_rows_equal(R.get()[-n:], wr, "add-back r x%d" % reps)
+    for cnt in (3000, 140000):
+        a2 = inputs.uniform(1100 + nb, cnt, 2 * nb)
+        b2 = inputs.uniform(1200 + nb, cnt, nb)
+        b2[:, -1] |= np.uint32(0x80000000)
+        A, B = handle.from_array(a2), handle.from_array(b2)
+        Q, R = handle.integers(64 * nb, cnt), handle.integers(32 * nb, cnt)
+        handle.divmod(Q, R, A, B)
+        rows = np.unique(np.linspace(0, cnt - 1, 1000).astype(int))
+        wq, wr = gmp(O.GO_DIV, rows.size, 2 * nb, a2[rows], b2[rows], out2_limbs=nb)
+        assert_rows_equal(Q.get()[rows], wq, "q, %d divisions" % cnt)
+        assert_rows_equal(R.get()[rows], wr, "r, %d divisions" % cnt)

@@ -18,7 +18,7 @@ from xmp_amd import ext
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("bits,count", [(512, 256), (1024, 512)])
+@pytest.mark.parametrize("bits,count", [(512, 256), (1024, 512), (2048, 256)])
 def test_tuner_run_load_and_choice(handle, bits, count):
     text, n = ext.tune(handle, bits, bits, count)
     rows = json.loads("[" + text + "]")

@@ -32,12 +32,26 @@ namespace xmp {
 namespace {
 
 // dividend sources ----------------------------------------------------------
+// Limb k of the dividend, 0 outside [0, len): one unconditional load at a
+// clamped, in-range limb and a select, with no branch around the load, so
+// the loads the division loops issue ahead of their use stay in flight
+// across loop iterations (a load under a branch made the compiler wait for
+// every load at the loop's back edge).
+// A limb fetched ahead of its use: `v & m` is the limb (m = 0 outside the
+// dividend), applied only when the limb is used, so the load stays in flight.
+struct RawLimb {
+  uint32_t v, m;
+  __device__ __forceinline__ uint32_t get() const { return v & m; }
+};
+
 struct OperandDividend {
   const uint32_t *data;
   uint32_t stride, inst, len;
-  __device__ __forceinline__ uint32_t operator()(int32_t k) const {
-    return (k >= 0 && (uint32_t)k < len) ? data[(size_t)k * stride + inst] : 0u;
+  __device__ __forceinline__ RawLimb raw(int32_t k) const {
+    const int32_t kc = k < 0 ? 0 : ((uint32_t)k < len ? k : (int32_t)len - 1);
+    return RawLimb{data[(size_t)kc * stride + inst], (k >= 0 && (uint32_t)k < len) ? ~0u : 0u};
   }
+  __device__ __forceinline__ uint32_t operator()(int32_t k) const { return raw(k).get(); }
   __device__ __forceinline__ uint32_t length() const { return len; }
 };
 
@@ -46,8 +60,18 @@ struct PowerOfTwoDividend {   // 2^e
   __device__ __forceinline__ uint32_t operator()(int32_t k) const {
     return (k >= 0 && (uint32_t)k == (e >> 5)) ? (1u << (e & 31)) : 0u;
   }
+  __device__ __forceinline__ RawLimb raw(int32_t k) const { return RawLimb{(*this)(k), ~0u}; }
   __device__ __forceinline__ uint32_t length() const { return (e >> 5) + 1; }
 };
+
+// limb g of an interleaved operand, 0 outside [0, len): an unconditional
+// load at a clamped, in-range limb and a select (see OperandDividend)
+__device__ __forceinline__ uint32_t limb_or_zero(const uint32_t *data, uint32_t stride, uint32_t inst, uint32_t len,
+                                                 int32_t g) {
+  const int32_t gc = g < 0 ? 0 : ((uint32_t)g < len ? g : (int32_t)len - 1);
+  const uint32_t v = data[(size_t)gc * stride + inst];
+  return (g >= 0 && (uint32_t)g < len) ? v : 0u;
+}
 
 __device__ __forceinline__ uint32_t shl_limb(uint32_t hi, uint32_t lo, uint32_t s) {
   // low 32 bits of ((hi:lo) << s) >> 32, i.e. (hi << s) | (lo >> (32-s)); s in [0,31]
@@ -146,7 +170,7 @@ __device__ __forceinline__ void divide_reg(const Dividend &A, const uint32_t *bd
   int top = -1;
 #pragma unroll
   for (int k = 0; k < NB; k++) {
-    v[k] = (uint32_t)k < blen ? bdata[(size_t)k * bstride + binst] : 0u;
+    v[k] = limb_or_zero(bdata, bstride, binst, blen, k);
     if (v[k]) top = k;
   }
   const uint32_t la = A.length();
@@ -167,19 +191,15 @@ __device__ __forceinline__ void divide_reg(const Dividend &A, const uint32_t *bd
   if (sl) {
 #pragma unroll
     for (int k = 0; k < NB; k++) {
-      int src = k - sl;
-      v[k] = (src >= 0 && (uint32_t)src < blen) ? bdata[(size_t)src * bstride + binst] : 0u;
+      v[k] = limb_or_zero(bdata, bstride, binst, blen, k - sl);
     }
   }
 #pragma unroll
   for (int k = NB - 1; k >= 0; k--) v[k] = shl_limb(v[k], k > 0 ? v[k - 1] : 0u, s);
   const uint32_t dinv = reciprocal(v[NB - 1]);
 
-  // normalised dividend U = A << (32*sl + s); U limb k:
-  auto U = [&](int32_t k) -> uint32_t {
-    int32_t x = k - sl;
-    return shl_limb(A(x), A(x - 1), s);
-  };
+  // normalised dividend U = A << (32*sl + s): U limb k is the funnel of A
+  // limbs k-sl and k-sl-1
   const int32_t lu = (int32_t)la + sl + 1;   // limbs of U
   const int32_t nq = lu - NB;                // quotient limbs computed
   if (nq <= 0) {
@@ -194,18 +214,28 @@ __device__ __forceinline__ void divide_reg(const Dividend &A, const uint32_t *bd
   // multiply-subtract writes the next (shifted) window, so no limb is moved
   uint32_t Wa[NB + 1], Wb[NB + 1];
 #pragma unroll
-  for (int k = 0; k <= NB; k++) Wa[k] = U(nq - 1 + k);
-  // the dividend limbs that enter the window, read four digits ahead
-  uint32_t n0 = U(nq - 2), n1 = U(nq - 3), n2 = U(nq - 4), n3 = U(nq - 5);
+  for (int k = 0; k <= NB; k++) Wb[k] = A(nq - 2 - sl + k);   // raw limbs, every load issued before any use
+  {
+    const uint32_t raw_top = A(nq - 1 - sl + NB);
+#pragma unroll
+    for (int k = 0; k <= NB; k++) Wa[k] = shl_limb(k < NB ? Wb[k + 1] : raw_top, Wb[k], s);
+  }
+  // The dividend limbs that enter the window: digit j takes U(j-1), the
+  // funnel of raw limbs j-1-sl and j-2-sl.  Raw limbs are loaded six ahead
+  // and only shifted when used, so their loads stay in flight for two
+  // iterations of the loop.
+  RawLimb r0 = A.raw(nq - 2 - sl), r1 = A.raw(nq - 3 - sl), r2 = A.raw(nq - 4 - sl), r3 = A.raw(nq - 5 - sl),
+          r4 = A.raw(nq - 6 - sl), r5 = A.raw(nq - 7 - sl);
   int32_t j = nq - 1;
   for (; j >= 1; j -= 2) {
-    digit_step<NB>(Wa, Wb, n0, v, dinv, o, j);
-    digit_step<NB>(Wb, Wa, n1, v, dinv, o, j - 1);
-    // digits j-2 and j-3 take U(j-3) and U(j-4); U(j-5), U(j-6) are read for the pair after
-    n0 = n2;
-    n1 = n3;
-    n2 = U(j - 5);
-    n3 = U(j - 6);
+    digit_step<NB>(Wa, Wb, shl_limb(r0.get(), r1.get(), s), v, dinv, o, j);
+    digit_step<NB>(Wb, Wa, shl_limb(r1.get(), r2.get(), s), v, dinv, o, j - 1);
+    r0 = r2;
+    r1 = r3;
+    r2 = r4;
+    r3 = r5;
+    r4 = A.raw(j - 7 - sl);
+    r5 = A.raw(j - 8 - sl);
   }
   if (j == 0) {
     digit_step<NB>(Wa, Wb, 0u, v, dinv, o, 0);
@@ -231,6 +261,248 @@ __device__ __forceinline__ void divide_reg(const Dividend &A, const uint32_t *bd
     for (uint32_t k = (uint32_t)(NB - sl > 0 ? NB - sl : 0); k < o.r_len; k++)
       o.r[(size_t)k * o.r_stride + o.r_inst] = 0u;
   }
+}
+
+// ---- two lanes per division ----------------------------------------------
+// Knuth D with the NB = 2H limb divisor and the remainder window split over
+// a pair of lanes (lane h of the pair holds limbs h*H .. h*H+H-1, lane 1 also
+// the window's top limb NB).  A batch too small to give every SIMD two waves
+// at one lane per division (65,536 divisions fill only one wave per SIMD of
+// the 1,024) gets twice the waves this way, and each lane issues about half
+// the instructions of a quotient digit.  Per digit:
+//  - lane 1 holds the window's top limbs and the divisor's top limbs, so it
+//    estimates q (as the one-lane kernel does) and both lanes take its value;
+//  - each lane subtracts q*V over its own limbs with two borrow chains; lane 1
+//    starts its high-word chain from hi(q * v[H-1]) of lane 0 (kept since the
+//    set-up), and then subtracts lane 0's outgoing borrows (0..2) with a
+//    ripple that only continues past its first limb in the rare case that
+//    limb underflows (a uniform branch on a wave ballot);
+//  - the rare add-back adds V on both lanes and ripples lane 0's carry into
+//    lane 1 the same way;
+//  - the window moves up one limb: lane 1's first limb takes lane 0's last.
+// Moves inside the pair are DPP quad_perm moves: [1,1,3,3] = 0xF5 gives both
+// lanes lane 1's value, [0,0,2,2] = 0xA0 lane 0's.  A move reads 0 from a
+// lane that is switched off, so every move runs on both lanes of the pair
+// (never under a condition on h) and its result is selected afterwards.
+__device__ __forceinline__ uint32_t pair_hi(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xF5, 0xF, 0xF, true);
+}
+__device__ __forceinline__ uint32_t pair_lo(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xA0, 0xF, 0xF, true);
+}
+
+template <int H>
+struct PairDiv {
+  static constexpr int NB = 2 * H;
+  uint32_t v[H];         // this lane's normalised divisor limbs
+  uint32_t vt, vt2, dinv;   // the divisor's two top limbs and the top limb's reciprocal (meaningful on lane 1)
+  uint32_t vbelow;       // lane 1: lane 0's top limb v[H-1]; lane 0: 0
+  uint32_t h;            // 0 or 1
+
+  // subtract c (< 2^32) from the limbs x[1..H] and then `top`: returns
+  // whether the subtraction borrowed out of the top.  The ripple past x[1]
+  // is rare, so it is a uniform branch on the wave's borrows.
+  __device__ __forceinline__ static bool sub_ripple(uint32_t (&x)[H + 1], uint32_t c, uint32_t &top) {
+    uint32_t br;
+    x[1] = __builtin_subc(x[1], c, 0u, &br);
+    if (__builtin_amdgcn_ballot_w64(br != 0) == 0) return false;
+#pragma unroll
+    for (int k = 2; k <= H; k++) x[k] = __builtin_subc(x[k], 0u, br, &br);
+    top = __builtin_subc(top, 0u, br, &br);
+    return br != 0;
+  }
+
+  // One quotient digit: Win (this lane's window limbs, lane 1's Win[H] the
+  // window top) minus q*V into Wout[1..H], Wout[0] = the next limb (lane 0:
+  // the dividend's next limb, lane 1: lane 0's Wout[H]).  Returns q.
+  __device__ __forceinline__ uint32_t step(const uint32_t (&Win)[H + 1], uint32_t (&Wout)[H + 1],
+                                           uint32_t next) const {
+    // estimate from the window's three top limbs (lane 1's), as the one-lane kernel
+    uint32_t qhat, rhat;
+    bool rhat_ok = true;
+    if (Win[H] >= vt) {
+      qhat = 0xFFFFFFFFu;
+      const uint64_t rr = (uint64_t)Win[H - 1] + vt;
+      rhat = (uint32_t)rr;
+      rhat_ok = (rr >> 32) == 0;
+    } else {
+      qhat = div_2by1(Win[H], Win[H - 1], vt, dinv, rhat);
+    }
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+      if (rhat_ok && (uint64_t)qhat * vt2 > (((uint64_t)rhat << 32) | Win[H - 2])) {
+        qhat--;
+        const uint64_t rr = (uint64_t)rhat + vt;
+        rhat = (uint32_t)rr;
+        rhat_ok = (rr >> 32) == 0;
+      }
+    }
+    uint32_t q = pair_hi(qhat);
+    // Win - q*V on this lane's limbs: limb k loses lo(q*v[k]) and hi(q*v[k-1])
+    uint32_t b1 = 0, b2 = 0, ph = (uint32_t)(((uint64_t)q * vbelow) >> 32);
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+      const uint64_t p = (uint64_t)q * v[k];
+      const uint32_t x = __builtin_subc(Win[k], (uint32_t)p, b1, &b1);
+      Wout[k + 1] = __builtin_subc(x, ph, b2, &b2);
+      ph = (uint32_t)(p >> 32);
+    }
+    const uint32_t out_borrow = b1 + b2;   // lane 0's borrows into lane 1's first limb
+    uint32_t top;
+    {
+      const uint32_t x = __builtin_subc(Win[H], ph, b1, &b1);
+      top = __builtin_subc(x, 0u, b2, &b2);
+    }
+    bool neg = (b1 | b2) != 0;
+    const uint32_t lo_borrow = pair_lo(out_borrow);   // every move unconditional: see pair_hi
+    const uint32_t c = h ? lo_borrow : 0u;
+    if (sub_ripple(Wout, c, top)) neg = true;
+    neg = pair_hi(neg ? 1u : 0u) != 0;   // lane 1's sign decides for the pair
+    if (__builtin_amdgcn_ballot_w64(neg) != 0) {
+      // the estimate was one too large (rare): add V back.  Both lanes of a
+      // pair take the same branch (neg is lane 1's), so the pair's move reads
+      // an active lane.
+      if (neg) {
+        q--;
+        uint32_t cy = 0;
+#pragma unroll
+        for (int k = 0; k < H; k++) Wout[k + 1] = __builtin_addc(Wout[k + 1], v[k], cy, &cy);
+        const uint32_t lo_carry = pair_lo(cy);
+        uint32_t cc = h ? lo_carry : 0u;   // lane 0's carry into lane 1's limbs
+#pragma unroll
+        for (int k = 0; k < H; k++) Wout[k + 1] = __builtin_addc(Wout[k + 1], 0u, cc, &cc);
+      }
+    }
+    const uint32_t from0 = pair_lo(Wout[H]);
+    Wout[0] = h ? from0 : next;
+    return q;
+  }
+};
+
+template <int H, class Dividend>
+__device__ __forceinline__ void divide_pair(const Dividend &A, const uint32_t *bdata, uint32_t bstride, uint32_t binst,
+                                            uint32_t blen, const DivOut &o, uint32_t h) {
+  constexpr int NB = 2 * H;
+  PairDiv<H> d;
+  d.h = h;
+  // the pair's top non-zero divisor limb
+  int top = -1;
+#pragma unroll
+  for (int k = 0; k < H; k++) {
+    const uint32_t g = h * H + (uint32_t)k;
+    d.v[k] = limb_or_zero(bdata, bstride, binst, blen, (int32_t)g);
+    if (d.v[k]) top = (int)g;
+  }
+  top = max(top, (int)pair_hi((uint32_t)top));   // lane 1's top when it has one, else lane 0's
+  top = max(top, (int)pair_lo((uint32_t)top));
+  const uint32_t la = A.length();
+  if (top < 0 || (int32_t)la + (NB - 1 - top) + 1 - NB <= 0) {
+    // division by zero (q = 0, r = a) or a dividend shorter than the
+    // normalised divisor (q = 0, r = a): lane 0 writes everything
+    if (h == 0) {
+      if (o.q)
+        for (uint32_t k = 0; k < o.q_len; k++) o.q[(size_t)k * o.q_stride + o.q_inst] = 0u;
+      if (o.r)
+        for (uint32_t k = 0; k < o.r_len; k++) o.r[(size_t)k * o.r_stride + o.r_inst] = A((int32_t)k);
+    }
+    return;
+  }
+  uint32_t vtop = 0;
+#pragma unroll
+  for (int k = 0; k < H; k++)
+    if ((int)(h * H + (uint32_t)k) == top) vtop = d.v[k];
+  vtop |= pair_hi(vtop) | pair_lo(vtop);
+  const uint32_t s = (uint32_t)__builtin_clz(vtop);
+  const int sl = NB - 1 - top;
+  // normalised divisor: limb g is (b << (32*sl + s)) limb g, read from memory
+  {
+    // raw limbs h*H-sl-1 .. h*H-sl+H-1 first (every load issued before any use)
+    uint32_t raw[H + 1];
+#pragma unroll
+    for (int k = 0; k <= H; k++) raw[k] = limb_or_zero(bdata, bstride, binst, blen, (int32_t)(h * H) - sl - 1 + k);
+#pragma unroll
+    for (int k = 0; k < H; k++) d.v[k] = shl_limb(raw[k + 1], raw[k], s);
+  }
+  d.vt = pair_hi(d.v[H - 1]);
+  d.vt2 = pair_hi(d.v[H - 2]);
+  d.dinv = reciprocal(d.vt);
+  const uint32_t lo_top = pair_lo(d.v[H - 1]);
+  d.vbelow = h ? lo_top : 0u;
+
+  const int32_t nq = (int32_t)la + sl + 1 - NB;   // quotient limbs computed (>= 1)
+  uint32_t Wa[H + 1], Wb[H + 1];
+  // window = U[nq-1 .. nq-1+NB]: lane h's limbs from h*H (lane 0's Wa[H] duplicates lane 1's Wa[0])
+#pragma unroll
+  for (int k = 0; k <= H; k++) Wb[k] = A(nq - 2 - sl + (int32_t)(h * H) + k);   // raw limbs first
+  {
+    const uint32_t raw_top = A(nq - 1 - sl + (int32_t)(h * H) + H);
+#pragma unroll
+    for (int k = 0; k <= H; k++) Wa[k] = shl_limb(k < H ? Wb[k + 1] : raw_top, Wb[k], s);
+  }
+  const bool store_q = o.q && h == 1;
+  // the dividend limbs that enter the window (lane 0's), raw limbs loaded six
+  // ahead and shifted when used (as in the one-lane kernel)
+  RawLimb r0 = A.raw(nq - 2 - sl), r1 = A.raw(nq - 3 - sl), r2 = A.raw(nq - 4 - sl), r3 = A.raw(nq - 5 - sl),
+          r4 = A.raw(nq - 6 - sl), r5 = A.raw(nq - 7 - sl);
+  int32_t j = nq - 1;
+  for (; j >= 1; j -= 2) {
+    uint32_t q = d.step(Wa, Wb, shl_limb(r0.get(), r1.get(), s));
+    if (store_q && (uint32_t)j < o.q_len) o.q[(size_t)j * o.q_stride + o.q_inst] = q;
+    q = d.step(Wb, Wa, shl_limb(r1.get(), r2.get(), s));
+    if (store_q && (uint32_t)(j - 1) < o.q_len) o.q[(size_t)(j - 1) * o.q_stride + o.q_inst] = q;
+    r0 = r2;
+    r1 = r3;
+    r2 = r4;
+    r3 = r5;
+    r4 = A.raw(j - 7 - sl);
+    r5 = A.raw(j - 8 - sl);
+  }
+  if (j == 0) {
+    const uint32_t q = d.step(Wa, Wb, 0u);
+    if (store_q && 0u < o.q_len) o.q[o.q_inst] = q;
+#pragma unroll
+    for (int k = 0; k <= H; k++) Wa[k] = Wb[k];
+  }
+  if (o.q && h == 0)
+    for (uint32_t k = (uint32_t)nq; k < o.q_len; k++) o.q[(size_t)k * o.q_stride + o.q_inst] = 0u;
+  if (o.r) {
+    // remainder limbs h*H + k are Wa[k + 1] (the last step moved them up one
+    // limb); unnormalise by s bits (lane 0's last limb takes lane 1's first)
+    // and sl limbs
+    const uint32_t hi_first = pair_hi(Wa[1]);
+    const uint32_t next_hi = h ? 0u : hi_first;
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+      const uint32_t hi = k + 1 < H ? Wa[k + 2] : next_hi;
+      const uint32_t val = s ? funnel(hi, Wa[k + 1], s) : Wa[k + 1];
+      const int32_t dst = (int32_t)(h * H) + k - sl;
+      if (dst >= 0 && (uint32_t)dst < o.r_len) o.r[(size_t)dst * o.r_stride + o.r_inst] = val;
+    }
+    if (h == 0)
+      for (uint32_t k = (uint32_t)(NB - sl); k < o.r_len; k++) o.r[(size_t)k * o.r_stride + o.r_inst] = 0u;
+  }
+}
+
+template <int H>
+__global__ void __launch_bounds__(256) divmod_pair_kernel(OutOperand q, OutOperand r, Operand a, Operand b,
+                                                          uint32_t count) {
+  // pairs of lanes; a pair past `count` takes division count-1 and writes the same values
+  const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t h = gt & 1u;
+  const uint32_t pt = gt >> 1;
+  const uint32_t t = pt < count ? pt : count - 1;
+  const uint32_t ai = in_index(a, t), bi = in_index(b, t);
+  DivOut o;
+  o.q = q.data;
+  o.q_len = q.len;
+  o.q_stride = q.stride;
+  o.q_inst = q.data ? out_index(q, t) : 0;
+  o.r = r.data;
+  o.r_len = r.len;
+  o.r_stride = r.stride;
+  o.r_inst = r.data ? out_index(r, t) : 0;
+  OperandDividend A{a.data, a.stride, ai, a.len};
+  divide_pair<H>(A, b.data, b.stride, bi, b.len, o, h);
 }
 
 template <int NB>
@@ -418,9 +690,24 @@ size_t divmod_scratch_words(uint32_t b_len, uint32_t count) {
   return (size_t)(2 * b_len + 1) * round_up(count, 256);
 }
 
+template <int H>
+hipError_t launch_divmod_pair(OutOperand q, OutOperand r, Operand a, Operand b, uint32_t count, hipStream_t stream) {
+  hipLaunchKernelGGL(divmod_pair_kernel<H>, dim3(div_round_up(2 * count, 256)), dim3(256), 0, stream, q, r, a, b,
+                     count);
+  return hipGetLastError();
+}
+
 hipError_t launch_divmod_ws(OutOperand q, OutOperand r, Operand a, Operand b, uint32_t count, uint32_t *scratch,
                             hipStream_t stream) {
   const uint32_t nb = b.len;
+  // Below 131,072 divisions one lane each gives fewer than two waves per
+  // SIMD (1,024 SIMDs x 2 x 64 lanes): two lanes per division instead
+  if (count < 131072 && nb > 16 && nb <= 128) {
+    if (nb <= 32) return launch_divmod_pair<16>(q, r, a, b, count, stream);
+    if (nb <= 64) return launch_divmod_pair<32>(q, r, a, b, count, stream);
+    if (nb <= 96) return launch_divmod_pair<48>(q, r, a, b, count, stream);
+    return launch_divmod_pair<64>(q, r, a, b, count, stream);
+  }
   if (nb <= 1) return launch_divmod_reg<1>(q, r, a, b, count, stream);
   if (nb <= 2) return launch_divmod_reg<2>(q, r, a, b, count, stream);
   if (nb <= 4) return launch_divmod_reg<4>(q, r, a, b, count, stream);

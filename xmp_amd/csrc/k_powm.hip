@@ -422,6 +422,8 @@ hipError_t launch_powm_latency(int lanes, int digits, const PowmArgs &args, hipS
     hipLaunchKernelGGL((powm_kernel<8, 10, 28>), grid, block, 0, stream, args);
   else if (lanes == 16 && digits == 5)
     hipLaunchKernelGGL((powm_kernel<16, 5, 28>), grid, block, 0, stream, args);
+  else if (lanes == 16 && digits == 3)
+    hipLaunchKernelGGL((powm_kernel<16, 3, 28>), grid, block, 0, stream, args);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -465,6 +467,7 @@ const VariantEntry kVariants[] = {
     XMP_LATENCY_VARIANT(8, 5, 28),   // 1120 : 1024-bit on eight lanes
     XMP_LATENCY_VARIANT(8, 10, 28),  // 2240 : 2048-bit on eight lanes
     XMP_LATENCY_VARIANT(16, 5, 28),  // 2240 : 2048-bit on sixteen lanes
+    XMP_LATENCY_VARIANT(16, 3, 28),  // 1344 : 1024-bit on sixteen lanes
 };
 #undef XMP_VARIANT
 #undef XMP_LATENCY_VARIANT
@@ -594,8 +597,8 @@ XMP_RUN(17, 32, 20, 26)
 XMP_RUN(18, 64, 20, 26)
 #undef XMP_RUN
 // Variant 9 (2x55, which spills as a called function) and the low-latency
-// layouts 19..22 are not part of the persistent kernel.
-static_assert(kNumVariants == 23, "run_variant<> must list every batched variant in kVariants order");
+// layouts 19..23 are not part of the persistent kernel.
+static_assert(kNumVariants == 24, "run_variant<> must list every batched variant in kVariants order");
 constexpr bool batched_variant(int i) { return i >= 0 && i < 19 && i != 9; }
 
 __global__ void __launch_bounds__(64, 2) powm_batch_kernel(BatchArgs ba) {

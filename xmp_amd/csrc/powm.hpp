@@ -51,7 +51,7 @@ inline uint32_t variant_rbits(const PowmVariant &v) { return (uint32_t)(v.lanes 
 size_t powm_table_words(const PowmVariant &v, uint32_t count, uint32_t window_bits);
 // launch the exponentiation kernel for one variant
 hipError_t launch_powm_variant(int variant, const PowmArgs &args, hipStream_t stream);
-// the few-digit latency layouts (4x10, 8x5, 8x10, 16x5 at radix 2^28), built in
+// the few-digit latency layouts (4x10, 8x5, 8x10, 16x5, 16x3 at radix 2^28), built in
 // k_powm_latency.hip; launch_powm_variant forwards to it
 hipError_t launch_powm_latency(int lanes, int digits, const PowmArgs &args, hipStream_t stream);
 // R^2 mod m (R = 2^rbits of the variant) for m[:, t], t < count, canonical, into

@@ -57,10 +57,13 @@ struct Request {
   Clock::time_point arrival;
 };
 
-// a completed request; its result words are in the server's result queue
-struct Done {
-  uint64_t tag;
-  double latency_us;
+// a finished batch's requests in slot order: tags, latencies and results
+// (n32 words each); Poll consumes them from `next` on
+struct DoneChunk {
+  std::vector<uint64_t> tags;
+  std::vector<double> latency_us;
+  std::vector<uint32_t> words;
+  size_t next = 0;
 };
 
 // device array in the interleaved layout
@@ -124,9 +127,7 @@ struct _xmpServer_t {
   std::deque<Request> pending;
   std::deque<uint32_t> free_slots;      // slots the scheduler may fill
   std::vector<uint32_t> launched;       // slots on the device, for the completer
-  std::deque<Done> done;                // completed requests in completion order ...
-  std::vector<uint32_t> done_words;     // ... and their results (n32 words each from done_head)
-  size_t done_head = 0;
+  std::deque<DoneChunk> done;           // completed batches in completion order
   uint64_t submitted = 0, completed = 0, batches = 0;
   uint32_t inflight = 0, peak_inflight = 0, peak_batch = 0;
   bool stopping = false, scheduler_exited = false;
@@ -186,7 +187,7 @@ hipError_t order(hipStream_t earlier, hipStream_t later, hipEvent_t ev) {
 // input buffer and exporting its results into the pinned output buffer, then
 // the completion event once both have finished.  The exponentiations run on
 // the slot's two main streams and everything else on its short streams.
-hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
+hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl, uint32_t scale) {
   XMP_TRACE("xmpServer batch");
   const uint32_t n32 = s->n32;
   const uint32_t na = sl.na, nb = sl.nb, cap = s->cap;
@@ -230,7 +231,7 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
     a.base = op(sl.a_base);
     a.exp = op(sl.a_exp);
     a.mod = op(s->N, d_keyA, na);
-    const _xmpServer_t::Layout &ly = pick_layout(s->lay_n, na);
+    const _xmpServer_t::Layout &ly = pick_layout(s->lay_n, na * scale);
     a.r2 = ly.r2;
     a.r2_stride = ly.r2_stride;
     a.table = sl.table_a;
@@ -258,7 +259,7 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl) {
     a.base = op(sl.b_cm);
     a.exp = op(s->DPQ, d_keyPQ, 2 * nb);
     a.mod = op(s->PQ, d_keyPQ, 2 * nb);
-    const _xmpServer_t::Layout &ly = pick_layout(s->lay_pq, 2 * nb);
+    const _xmpServer_t::Layout &ly = pick_layout(s->lay_pq, 2 * nb * scale);
     a.r2 = ly.r2;
     a.r2_stride = ly.r2_stride;
     a.table = sl.table_b;
@@ -335,6 +336,9 @@ void scheduler_loop(xmpServer_t s) {
     sl.nb = (uint32_t)b.size();
     for (auto &r : b) sl.reqs.push_back(std::move(r));
     s->inflight += take;
+    // the layout is chosen as for a batch `scale` times this one: the slots in
+    // flight share the device (or the configured factor)
+    const uint32_t scale = s->cfg.layout_scale ? s->cfg.layout_scale : (uint32_t)s->launched.size() + 1;
     s->peak_inflight = std::max(s->peak_inflight, s->inflight);
     s->peak_batch = std::max(s->peak_batch, take);
     s->batches++;
@@ -346,7 +350,7 @@ void scheduler_loop(xmpServer_t s) {
     }
     for (uint32_t j = 0; j < sl.nb; j++)
       std::memcpy(sl.h_in + 2 * words + (size_t)j * n32, sl.reqs[sl.na + j].base.data(), n32 * 4);
-    const bool ok = launch_batch(s, sl) == hipSuccess;
+    const bool ok = launch_batch(s, sl, scale) == hipSuccess;
     // part of a failed batch may be enqueued: wait for all of it before the
     // slot's buffers can be reused
     if (!ok) sync_slot(sl);
@@ -381,28 +385,33 @@ void completer_loop(xmpServer_t s) {
       if (q == hipErrorNotReady) continue;
       progressed = true;
       const auto now = Clock::now();
-      if (q != hipSuccess) sync_slot(sl);
-      {
+      if (q != hipSuccess) {
+        // the batch failed on the device: its results are not published
+        sync_slot(sl);
         std::lock_guard<std::mutex> lk(s->mu);
         s->launched.erase(std::find(s->launched.begin(), s->launched.end(), slot));
-        if (q != hipSuccess) {
-          // the batch failed on the device: its results are not published
-          fail_slot_locked(s, slot);
-        } else {
-          const size_t words = (size_t)s->cap * n32;
-          if (s->done_head > 0 && s->done_head * 2 >= s->done_words.size()) {
-            s->done_words.erase(s->done_words.begin(), s->done_words.begin() + (ptrdiff_t)s->done_head);
-            s->done_head = 0;
-          }
-          s->done_words.insert(s->done_words.end(), sl.h_out, sl.h_out + (size_t)sl.na * n32);
-          s->done_words.insert(s->done_words.end(), sl.h_out + words, sl.h_out + words + (size_t)sl.nb * n32);
-          for (const Request &r : sl.reqs)
-            s->done.push_back({r.tag, std::chrono::duration<double, std::micro>(now - r.arrival).count()});
-          s->completed += sl.reqs.size();
-          s->inflight -= (uint32_t)sl.reqs.size();
-          sl.reqs.clear();
-          s->free_slots.push_back(slot);
+        fail_slot_locked(s, slot);
+      } else {
+        // the results are copied out before the lock is taken, so that
+        // submitters and the scheduler never wait for the copy
+        DoneChunk c;
+        const size_t words = (size_t)s->cap * n32, nreq = sl.reqs.size();
+        c.words.resize(nreq * n32);
+        std::memcpy(c.words.data(), sl.h_out, sizeof(uint32_t) * (size_t)sl.na * n32);
+        std::memcpy(c.words.data() + (size_t)sl.na * n32, sl.h_out + words, sizeof(uint32_t) * (size_t)sl.nb * n32);
+        c.tags.resize(nreq);
+        c.latency_us.resize(nreq);
+        for (size_t i = 0; i < nreq; i++) {
+          c.tags[i] = sl.reqs[i].tag;
+          c.latency_us[i] = std::chrono::duration<double, std::micro>(now - sl.reqs[i].arrival).count();
         }
+        sl.reqs.clear();   // the slot is not free yet: no other thread touches it
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->launched.erase(std::find(s->launched.begin(), s->launched.end(), slot));
+        s->done.push_back(std::move(c));
+        s->completed += nreq;
+        s->inflight -= (uint32_t)nreq;
+        s->free_slots.push_back(slot);
       }
       s->cv_sched.notify_one();
       s->cv_drain.notify_all();
@@ -502,8 +511,11 @@ xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const
       const uint32_t pbits = kind == 0 ? bits : bits / 2, mult = kind == 0 ? 1 : 2;
       xmpIntegers_t mods = kind == 0 ? s->N : s->PQ;
       std::vector<uint32_t> counts;
-      for (uint32_t c = 64; c < cap; c *= 4) counts.push_back(c * mult);
-      counts.push_back(cap * mult);
+      // up to the largest batch size a layout is chosen for: the capacity
+      // times the layout scale (at most the number of slots)
+      const uint32_t top = cap * (config->layout_scale ? config->layout_scale : config->streams);
+      for (uint32_t c = 64; c < top; c *= 4) counts.push_back(c * mult);
+      counts.push_back(top * mult);
       for (uint32_t c : counts) {
         PowmChoice ch;
         xmpError_t err = choose_powm(handle, pbits, pbits, c, xmpAlgorithmDefault, &ch);
@@ -635,18 +647,15 @@ xmpError_t XMPAPI xmpServerPoll(xmpServer_t server, uint32_t max, uint64_t *tags
   std::lock_guard<std::mutex> lk(server->mu);
   uint32_t k = 0;
   while (k < max && !server->done.empty()) {
-    const Done &d = server->done.front();
-    if (tags) tags[k] = d.tag;
+    DoneChunk &c = server->done.front();
+    const size_t n = std::min<size_t>(max - k, c.tags.size() - c.next);
+    if (tags) std::memcpy(tags + k, c.tags.data() + c.next, sizeof(uint64_t) * n);
     if (results)
-      std::memcpy(results + (size_t)k * n32, server->done_words.data() + server->done_head, sizeof(uint32_t) * n32);
-    if (latency_us) latency_us[k] = d.latency_us;
-    server->done.pop_front();
-    server->done_head += n32;
-    k++;
-  }
-  if (server->done.empty()) {
-    server->done_words.clear();
-    server->done_head = 0;
+      std::memcpy(results + (size_t)k * n32, c.words.data() + c.next * n32, sizeof(uint32_t) * n * n32);
+    if (latency_us) std::memcpy(latency_us + k, c.latency_us.data() + c.next, sizeof(double) * n);
+    c.next += n;
+    k += (uint32_t)n;
+    if (c.next == c.tags.size()) server->done.pop_front();
   }
   *n_out = k;
   return server->worker_error;
@@ -694,20 +703,50 @@ xmpError_t XMPAPI xmpServerRunPoisson(xmpServer_t server, double rate_per_s, uin
   XMP_CHECK_NE(server, nullptr);
   if (rate_per_s <= 0 || requests == 0 || !kinds || !keys || !bases || !exps) return xmpErrorInvalidParameter;
   const uint32_t n32 = server->n32;
-  std::mt19937_64 rng(seed);
-  std::exponential_distribution<double> gap(rate_per_s);
-  auto t0 = Clock::now();
-  double t = 0;
-  for (uint32_t i = 0; i < requests; i++) {
-    t += gap(rng);
-    auto due = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(t));
-    while (Clock::now() < due) {
+  // the requests and their arrival offsets are prepared before the clock starts
+  std::vector<Request> reqs(requests);
+  std::vector<double> due(requests);
+  {
+    std::mt19937_64 rng(seed);
+    std::exponential_distribution<double> gap(rate_per_s);
+    double t = 0;
+    for (uint32_t i = 0; i < requests; i++) {
+      if (kinds[i] > 1 || keys[i] >= server->nkeys) return xmpErrorInvalidParameter;
+      t += gap(rng);
+      due[i] = t;
+      Request &r = reqs[i];
+      r.kind = kinds[i];
+      r.key = keys[i];
+      r.tag = i;
+      r.base.assign(bases + (size_t)i * n32, bases + (size_t)(i + 1) * n32);
+      if (r.kind == 0) r.exp.assign(exps + (size_t)i * n32, exps + (size_t)(i + 1) * n32);
     }
-    xmpError_t e = xmpServerSubmit(server, kinds[i], keys[i], bases + (size_t)i * n32, exps + (size_t)i * n32, i);
-    if (e != xmpErrorSuccess) return e;
+  }
+  // every request is submitted at its arrival time; the ones already due
+  // when the generator gets to them are submitted together, under one lock
+  const auto t0 = Clock::now();
+  uint32_t i = 0;
+  while (i < requests) {
+    const auto first = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(due[i]));
+    while (Clock::now() < first) {
+    }
+    const auto now = Clock::now();
+    const double elapsed = std::chrono::duration<double>(now - t0).count();
+    uint32_t end = i + 1;
+    while (end < requests && due[end] <= elapsed) end++;
+    {
+      std::lock_guard<std::mutex> lk(server->mu);
+      for (uint32_t k = i; k < end; k++) {
+        reqs[k].arrival = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(due[k]));
+        server->pending.push_back(std::move(reqs[k]));
+      }
+      server->submitted += end - i;
+    }
+    server->cv_sched.notify_one();
+    i = end;
   }
   xmpError_t e = xmpServerDrain(server);
-  auto t1 = Clock::now();
+  const auto t1 = Clock::now();
   if (wall_s) *wall_s = std::chrono::duration<double>(t1 - t0).count();
   std::vector<uint64_t> tags(requests);
   std::vector<uint32_t> res((size_t)requests * n32);
@@ -721,11 +760,11 @@ xmpError_t XMPAPI xmpServerRunPoisson(xmpServer_t server, double rate_per_s, uin
     if (k == 0) break;
     got += k;
   }
-  for (uint32_t i = 0; i < got; i++) {
-    uint64_t tg = tags[i];
+  for (uint32_t k = 0; k < got; k++) {
+    const uint64_t tg = tags[k];
     if (tg >= requests) continue;
-    if (results) std::memcpy(results + tg * n32, res.data() + (size_t)i * n32, sizeof(uint32_t) * n32);
-    if (latency_us) latency_us[tg] = lat[i];
+    if (results) std::memcpy(results + tg * n32, res.data() + (size_t)k * n32, sizeof(uint32_t) * n32);
+    if (latency_us) latency_us[tg] = lat[k];
   }
   return e != xmpErrorSuccess ? e : (got == requests ? xmpErrorSuccess : xmpErrorUnspecified);
 }

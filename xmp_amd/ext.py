@@ -126,7 +126,8 @@ def tuner_load(json_text):
 
 
 class ServerConfig(ctypes.Structure):
-    _fields_ = [("streams", _u32), ("max_batch", _u32), ("hbm_budget", ctypes.c_size_t), ("reserved_cus", _u32)]
+    _fields_ = [("streams", _u32), ("max_batch", _u32), ("hbm_budget", ctypes.c_size_t), ("reserved_cus", _u32),
+                ("layout_scale", _u32)]
 
 
 class ServerStats(ctypes.Structure):
@@ -191,14 +192,14 @@ def make_keys(nkeys, bits, seed):
 class Server:
     """Continuous-batching powm server over resident keys (xmpServerCreate)."""
 
-    def __init__(self, handle, keys, bits, streams=4, max_batch=1024, hbm_budget=0, reserved_cus=0):
+    def __init__(self, handle, keys, bits, streams=4, max_batch=1024, hbm_budget=0, reserved_cus=0, layout_scale=1):
         import numpy as np
         from . import check, lib
         L = lib()
         declare_server(L)
         self.handle, self.bits, self.n32 = handle, bits, bits // 32
         self.keys = {k: np.ascontiguousarray(v, np.uint32) for k, v in keys.items()}
-        cfg = ServerConfig(streams, max_batch, hbm_budget, reserved_cus)
+        cfg = ServerConfig(streams, max_batch, hbm_budget, reserved_cus, layout_scale)
         self.s = _vp()
         kk = self.keys
         check(L.xmpServerCreate(handle.h, ctypes.byref(self.s), ctypes.byref(cfg), bits, kk["n"].shape[0],
