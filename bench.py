@@ -452,11 +452,18 @@ def main():
         log("mixed batch: batched %.1f ms, plain %.1f ms (x%.2f)" % (
             mixed["batched_ms"], mixed["plain_ms"], mixed["batched_over_plain"]))
         # ---- serving: 1024 resident keys, Poisson arrivals
+        # the latency-oriented shape (4 slots of 8,192, each batch on the layout
+        # for its own size) gives the p50/p99 figures; the throughput-oriented
+        # shape (8 slots of 16,384, layouts chosen for all slots in flight)
+        # gives the highest saturation
         serving, serve_sample = run_serving(h, 1024, SERVE_REQUESTS, 99, reserved_cus=args.serve_reserved_cus,
                                             slots=args.serve_slots, batch=args.serve_batch,
                                             layout_scale=args.serve_layout_scale)
         extra["serving"] = serving
         log("serving: %s" % json.dumps(serving))
+        serving_tp, _ = run_serving(h, 1024, SERVE_REQUESTS, 99, slots=8, batch=16384, layout_scale=0)
+        extra["serving_throughput_shape"] = serving_tp
+        log("serving (throughput shape): %s" % json.dumps(serving_tp))
 
     # ---- CPU leg: GMP (the reference test suite's oracle) on this host's cores.
     # It times mpz_powm on a bounded sample of the same instances (the CPU

@@ -64,13 +64,26 @@ def test_powm_above_8192_bits_special_values(handle):
     assert_rows_equal(Out.get(), gmp(O.GO_POWM, n, limbs, b, e, m), "special values above 8192 bits")
 
 
-@pytest.mark.parametrize("bits,exp_bits,n", [(34816, 96, 2), (40960, 1024, 2), (65536, 64, 2), (131072, 32, 1)])
+@pytest.mark.parametrize("bits,exp_bits,n", [(34816, 96, 2), (40960, 1024, 2), (65536, 64, 2), (131072, 32, 1),
+                                             (262144, 32, 1), (393184, 32, 1)])
 def test_powm_wider_than_lane_layouts(handle, bits, exp_bits, n):
     """Moduli above 33,278 bits, beyond every lane layout: one workgroup per
-    instance (8, 16 and 32 digits per thread), R^2 mod m from a short division
-    and Montgomery squarings; against GMP."""
+    instance (256 threads with 8, 16 and 32 digits each, then 512 threads with
+    32), R^2 mod m from a short division and Montgomery squarings; against GMP."""
     got, b, e, m = _powm_case(handle, 3100 + bits + exp_bits, n, bits, exp_bits=exp_bits, reduce_base=False)
     assert_rows_equal(got, gmp(O.GO_POWM, n, bits // 32, b, e, m), "powm %d-bit, %d-bit exponent" % (bits, exp_bits))
+
+
+def test_powm_beyond_the_widest_tier(handle):
+    """Above 393,214 bits (the 512-thread tier's 16,384 radix-2^24 digits) no
+    layout holds the modulus: xmpIntegersPowm returns xmpErrorUnsupported
+    before launching anything."""
+    arrs = [handle.integers(393216, 1) for _ in range(3)] + [handle.integers(32, 1)]
+    m, b, o, e = arrs
+    rc = xmp_amd.lib().xmpIntegersPowm(handle.h, o.x, b.x, e.x, m.x, 1)
+    assert rc == xmp_amd.UNSUPPORTED
+    for x in arrs:
+        x.close()
 
 
 def test_powm_wider_than_lane_layouts_special_values(handle):

@@ -1,24 +1,27 @@
 // k_powm_wide.hip -- modular exponentiation for moduli wider than any lane
-// layout holds (above 33,278 bits): one 256-thread workgroup per instance.
+// layout holds (above 33,278 bits): one 256- or 512-thread workgroup per
+// instance.
 //
 // Reference: the digit-serial fallback the reference uses above 8192 bits
 // (src/include/powm/digitized.h, powm_implementations.h:154-286), with the
 // same fixed-window schedule as every other powm here (powm/powm.h:94-98).
 //
-// The instance's ND = 256*C radix-2^24 digits are spread over the four waves
-// of a workgroup, C per thread, with the CIOS accumulator as 64-bit lazy sums
-// in registers (2*ND products of < 2^48 plus carries stay below 2^64 for
-// ND <= 8192).  Per CIOS step every thread reads the step's multiplier digit
+// The instance's ND = T*C radix-2^24 digits are spread over the T threads
+// (four or eight waves) of a workgroup, C per thread, with the CIOS
+// accumulator as 64-bit lazy sums in registers (2*ND products of < 2^48 plus
+// carries stay below 2^64 for ND <= 16384).  Per CIOS step every thread reads the step's multiplier digit
 // from LDS, thread 0 publishes the quotient digit of the next step, and every
 // thread hands the low digit of its retired accumulator to the thread below,
 // through LDS with one workgroup barrier per step (the quotient and the
 // hand-over are double-buffered, so a step's reads never meet the next step's
 // writes).  R = 2^(24*ND) = 3 * 2^(11 + log2 C), so R^2 mod m comes from one
-// short division of 2^(rbits + 3) and 11 + log2 C Montgomery squarings.
+// short division of 2^(rbits + 3) and log2(T*C) + 3 Montgomery squarings.
 //
-// The layout trades speed for reach: each step pays a workgroup barrier, and
-// a modulus just above a tier pays the next tier's digit count.  It exists so
-// that no modulus size is left unsupported up to 196,606 bits.
+// Tiers (T x C): 256 x 8, 256 x 16, 256 x 32 and 512 x 32, i.e. moduli up to
+// 49,150, 98,302, 196,606 and 393,214 bits.  The 512-thread tier holds 16,384
+// digits in 64 KiB of LDS with 8 waves on one CU (two per SIMD).  The layout
+// trades speed for reach: each step pays a workgroup barrier, and a modulus
+// just above a tier pays the next tier's digit count.
 #include "devmp.hpp"
 #include "lanes.hpp"
 #include "powm.hpp"
@@ -29,10 +32,10 @@ namespace {
 
 constexpr int kWideD = 24;
 constexpr uint32_t kWideMask = (1u << kWideD) - 1;
-constexpr int kWideThreads = 256;
 
-template <int C>
+template <int C, int T>
 struct Wide {
+  static constexpr int kWideThreads = T;
   static constexpr int ND = kWideThreads * C;
   static_assert(C >= 2, "a thread's lowest accumulator must not depend on its neighbour");
   static_assert(2.0 * ND * (double)kWideMask * kWideMask + 4.0 * (1ull << 40) < 18446744073709551615.0,
@@ -105,10 +108,11 @@ struct Wide {
 };
 
 // Thread t holds digits t*C .. t*C+C-1; tables store digit t*C+k of entry e
-// at ((e * C + k) * 256 + t), so a table access is coalesced over threads.
-template <int C>
-__global__ void __launch_bounds__(kWideThreads) powm_wide_kernel(PowmArgs args, uint32_t squarings) {
-  using W = Wide<C>;
+// at ((e * C + k) * T + t), so a table access is coalesced over threads.
+template <int C, int T>
+__global__ void __launch_bounds__(T) powm_wide_kernel(PowmArgs args, uint32_t squarings) {
+  using W = Wide<C, T>;
+  constexpr int kWideThreads = T;
   constexpr int ND = W::ND;
   __shared__ uint32_t s_b[ND];
   __shared__ uint32_t s_x[2 * kWideThreads];
@@ -148,8 +152,8 @@ __global__ void __launch_bounds__(kWideThreads) powm_wide_kernel(PowmArgs args, 
   } else {
     const uint32_t bi = in_index(args.base, gi), ei = in_index(args.exp, gi);
     const uint32_t wbits = args.window_bits;
-    const uint32_t T = 1u << wbits;
-    uint32_t *tab = args.table + (size_t)li * T * ND;
+    const uint32_t entries = 1u << wbits;
+    uint32_t *tab = args.table + (size_t)li * entries * ND;
     auto tab_at = [&](uint32_t e, int k) -> uint32_t * { return tab + ((size_t)e * C + k) * kWideThreads + tid; };
     // slot 0 = R mod m, slot 1 = base R mod m, slot e = slot e-1 * slot 1
 #pragma unroll
@@ -169,7 +173,7 @@ __global__ void __launch_bounds__(kWideThreads) powm_wide_kernel(PowmArgs args, 
 #pragma unroll
     for (int k = 0; k < C; k++) *tab_at(1, k) = a[k];
     w.stage(a);
-    for (uint32_t e = 2; e < T; e++) {
+    for (uint32_t e = 2; e < entries; e++) {
       w.mul(a, a);
 #pragma unroll
       for (int k = 0; k < C; k++) *tab_at(e, k) = a[k];
@@ -229,34 +233,41 @@ __global__ void __launch_bounds__(kWideThreads) powm_wide_kernel(PowmArgs args, 
   }
 }
 
-template <int C>
+template <int C, int T>
 hipError_t launch_wide(const PowmArgs &args, uint32_t squarings, hipStream_t stream) {
-  hipLaunchKernelGGL(powm_wide_kernel<C>, dim3(args.count), dim3(kWideThreads), 0, stream, args, squarings);
+  hipLaunchKernelGGL((powm_wide_kernel<C, T>), dim3(args.count), dim3(T), 0, stream, args, squarings);
   return hipGetLastError();
+}
+
+// a tier is named by its digit count in units of 256 digits: 8, 16, 32 (256
+// threads with 8, 16, 32 digits each) and 64 (512 threads with 32)
+hipError_t launch_tier(uint32_t tier, const PowmArgs &args, uint32_t squarings, hipStream_t stream) {
+  switch (tier) {
+    case 8: return launch_wide<8, 256>(args, squarings, stream);
+    case 16: return launch_wide<16, 256>(args, squarings, stream);
+    case 32: return launch_wide<32, 256>(args, squarings, stream);
+    case 64: return launch_wide<32, 512>(args, squarings, stream);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace
 
 uint32_t powm_wide_digits_per_thread(uint32_t precision) {
-  for (uint32_t c : {8u, 16u, 32u})
-    if (kWideD * kWideThreads * c >= precision + 2) return c;
+  for (uint32_t tier : {8u, 16u, 32u, 64u})
+    if (kWideD * 256u * tier >= precision + 2) return tier;
   return 0;
 }
 
-uint32_t powm_wide_rbits(uint32_t c) { return kWideD * kWideThreads * c; }
+uint32_t powm_wide_rbits(uint32_t tier) { return kWideD * 256u * tier; }
 
-size_t powm_wide_table_words(uint32_t c, uint32_t count, uint32_t window_bits) {
-  return (size_t)count * ((size_t)1 << window_bits) * kWideThreads * c;
+size_t powm_wide_table_words(uint32_t tier, uint32_t count, uint32_t window_bits) {
+  return (size_t)count * ((size_t)1 << window_bits) * 256u * tier;
 }
 
-hipError_t launch_powm_wide(uint32_t c, const PowmArgs &args, hipStream_t stream) {
+hipError_t launch_powm_wide(uint32_t tier, const PowmArgs &args, hipStream_t stream) {
   if (args.count == 0) return hipErrorInvalidValue;
-  switch (c) {
-    case 8: return launch_wide<8>(args, 0, stream);
-    case 16: return launch_wide<16>(args, 0, stream);
-    case 32: return launch_wide<32>(args, 0, stream);
-    default: return hipErrorInvalidValue;
-  }
+  return launch_tier(tier, args, 0, stream);
 }
 
 hipError_t launch_wide_r2(uint32_t c, uint32_t *r2, uint32_t r2_stride, const uint32_t *m, uint32_t m_len,
@@ -274,13 +285,7 @@ hipError_t launch_wide_r2(uint32_t c, uint32_t *r2, uint32_t r2_stride, const ui
   a.base = Operand{r2, m_len, r2_stride, count, nullptr, 0};
   a.out = OutOperand{ws, m_len + 1, ws_stride, nullptr};
   a.count = count;
-  switch (c) {
-    case 8: e = launch_wide<8>(a, j, stream); break;
-    case 16: e = launch_wide<16>(a, j, stream); break;
-    case 32: e = launch_wide<32>(a, j, stream); break;
-    default: return hipErrorInvalidValue;
-  }
-  if (e != hipSuccess) return e;
+  if ((e = launch_tier(c, a, j, stream)) != hipSuccess) return e;
   return launch_r2_canonical(r2, r2_stride, ws, ws_stride, m, m_len, m_stride, count, stream);
 }
 
