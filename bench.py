@@ -221,6 +221,14 @@ def run_only(args):
         P2 = h.integers(4096, n2)
         fn = lambda: lib.xmpIntegersMulAsync(h.h, P2.x, A2.x, A2.x, n2)  # noqa: E731
         what = "mul_cols_kernel<74,74,28,square> 1,048,576 x 2048-bit squares -> 4096-bit"
+    elif args.only == "div":
+        a3 = inputs.uniform(4343, 65536, 128)
+        d3 = inputs.uniform(4344, 65536, 64)
+        d3[:, -1] |= np.uint32(0x80000000)
+        A3, D3 = h.from_array(a3), h.from_array(d3)
+        Q3 = h.integers(4096, 65536)
+        fn = lambda: lib.xmpIntegersDivAsync(h.h, Q3.x, A3.x, D3.x, 65536)  # noqa: E731
+        what = "divmod_pair_kernel<32> 65,536 x 4096-bit / 2048-bit (n(m-n) = 4096 canonical MACs per division)"
     elif args.only == "add":
         n2 = 1 << 20
         a2, b2 = inputs.ordered_pair(4242, n2, 64)
@@ -255,7 +263,7 @@ def main():
     ap.add_argument("--serve-batch", type=int, default=SERVE_BATCH, help="serving requests per slot")
     ap.add_argument("--serve-layout-scale", type=int, default=1,
                     help="serving layout chosen as for a batch this many times larger (0 = slots in flight)")
-    ap.add_argument("--only", choices=["powm", "add", "mul", "sqr", "copy", "mixed", "serve"],
+    ap.add_argument("--only", choices=["powm", "add", "mul", "sqr", "div", "copy", "mixed", "serve"],
                     help="run just one kernel (for counter profiling): the 2048-bit powm kernel with resident "
                          "constants, the 2048-bit add, or a plain 256 MiB import copy for counter calibration")
     args = ap.parse_args()
