@@ -264,7 +264,10 @@ def test_montgomery_r2_canonical(handle, bits, n):
     ctx = Montgomery(handle, M, bits, n)
     try:
         lanes, digits, radix, _ = ctx.layout()
-        rbits = lanes * digits * radix
+        # R spans the lanes the modulus needs: the fewest whole lanes of
+        # digits*radix bits that reach bits + 2
+        need = -(-(bits + 2) // radix)
+        rbits = digits * radix * min(lanes, -(-need // digits))
         got = ctx.r2()
     finally:
         ctx.close()

@@ -49,3 +49,34 @@ def test_tuner_run_load_and_choice(handle, bits, count):
     finally:
         with open(shipped) as f:
             assert xmp_amd.lib().xmpTunerLoad(f.read().encode()) == 0
+
+
+def _entry(bits, count, lanes, digits, ms, window=6):
+    return {"precision": bits, "exp_bits": bits, "count": count, "lanes": lanes, "digits": digits, "radix": 28,
+            "window": window, "ms": ms, "powm_per_s": count / (ms * 1e-3)}
+
+
+@pytest.mark.parametrize("count,want", [(1000, (16, 3, 6)), (4096, (16, 3, 6)), (8192, (4, 10, 6)),
+                                        (16384, (4, 10, 5)), (100000, (4, 10, 5))])
+def test_choice_between_measured_batch_sizes(handle, count, want):
+    """Between two measured batch sizes each layout's time is interpolated on a
+    log-log scale: 16x3 is faster at 4,096 (2.4 against 2.8 ms) and 4x10 at
+    16,384 (2.84 against 6.0 ms), so 8,192, halfway in log scale, goes to 4x10
+    (about 2.82 against 3.8 ms), with the window measured at 4,096 (ties go to
+    the smaller size); outside the measured sizes the nearest size decides."""
+    rows = [_entry(1024, 4096, 16, 3, 2.4, 6), _entry(1024, 4096, 4, 10, 2.8, 6),
+            _entry(1024, 16384, 16, 3, 6.0, 6), _entry(1024, 16384, 4, 10, 2.84, 5)]
+    shipped = os.path.join(os.path.dirname(xmp_amd.LIB_PATH), "tune_gfx950.json")
+    M = handle.from_array(inputs.odd_moduli(3400, 64, 32))
+    try:
+        assert xmp_amd.lib().xmpTunerLoad(json.dumps({"device": "gfx950", "entries": rows}).encode()) == 0
+        ctx = ext.Montgomery(handle, M, 1024, count)
+        try:
+            lanes, digits, radix, window = ctx.layout()
+        finally:
+            ctx.close()
+        assert (lanes, digits, window) == want
+    finally:
+        M.close()
+        with open(shipped) as f:
+            assert xmp_amd.lib().xmpTunerLoad(f.read().encode()) == 0

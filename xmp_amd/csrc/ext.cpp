@@ -248,8 +248,11 @@ xmpError_t XMPAPI xmpTunerRun(xmpHandle_t handle, uint32_t precision, uint32_t e
   for (int vi = 0; vi < powm_variant_count(); vi++) {
     PowmVariant v = powm_variant(vi);
     if (!variant_fits(v, precision)) continue;
-    // a layout more than twice the size's width only wastes multiply-adds
-    if (variant_rbits(v) > 2 * precision + 128) continue;
+    // a layout much wider than the size leaves most of its lanes without
+    // digits (its steps stop at the lanes that hold them, but the idle lanes
+    // still take wave slots); up to three times the width can still win on
+    // small batches, where the extra lanes fill otherwise idle SIMDs
+    if (variant_rbits(v) > 3 * precision + 128) continue;
     uint32_t *r2 = nullptr, r2s = 0;
     if ((e = montgomery_prepare(handle, M, vi, &r2, &r2s)) != xmpErrorSuccess) break;
     for (uint32_t w = (w0 > 2 ? w0 - 1 : 1); w <= w0 + 1 && w <= 8; w++) {

@@ -17,9 +17,10 @@
 //    single v_mad_u64_u32 into its 64-bit accumulator and no carry is
 //    propagated inside the multiplication (2*ND products of < 2^(2D) each fit
 //    in 64 bits);
-//  * R = 2^(D*ND) >= 4m, so every product of inputs < 2m is < 2m again
-//    (Walter's bound) and no conditional subtraction is needed until the
-//    final conversion;
+//  * R = 2^(D*K*nl) >= 4m, nl <= L the lanes the modulus' digits occupy, so
+//    every product of inputs < 2m is < 2m again (Walter's bound) and no
+//    conditional subtraction is needed until the final conversion; a
+//    product's steps run over those nl*K digits only;
 //  * an instance spans L lanes of the wave (L = 1 ... 64), each holding K
 //    digits of the multiplicand, the modulus and the accumulator in
 //    registers.  Per CIOS step the quotient digit is broadcast from the
@@ -74,6 +75,9 @@ struct Mont {
   Group<L> g;
   uint32_t m[K];
   uint32_t np0;
+  // lanes whose digits the modulus occupies (powm_active_lanes): the steps
+  // of a product run over their nl*K digits, R = 2^(D*K*nl)
+  uint32_t nl = L;
   // the digit mask held in a register: an AND with a register operand can
   // take a DPP move of its other operand as a modifier, so the quotient's
   // broadcast and the digit hand-over each fold into their mask instruction
@@ -101,21 +105,32 @@ struct Mont {
     t[0] += f >> D;
   }
 
-  // r = a * b / R (mod m), r < 2m, normalised digits.  b: group's ND digits in LDS.
+  // r = a * b / R (mod m), r < 2m, normalised digits.  b: group's ND digits in
+  // LDS, of which the first nl*K are the multiplier's (R = 2^(D*K*nl)).  With
+  // every lane's digits the step count is a constant, as the compiler best
+  // schedules it; otherwise a run-time count.
   __device__ __forceinline__ void mul(uint32_t (&r)[K], const uint32_t (&a)[K], const uint32_t *b) const {
+    if (L == 1 || nl == (uint32_t)L)
+      mul_steps(r, a, b, ND);
+    else
+      mul_steps(r, a, b, (int)nl * K);
+  }
+
+  __device__ __forceinline__ void mul_steps(uint32_t (&r)[K], const uint32_t (&a)[K], const uint32_t *b,
+                                            const int ns) const {
     uint64_t t[K];
 #pragma unroll
     for (int k = 0; k < K; k++) t[k] = 0;
-    constexpr int ND4 = ND / 4 * 4;
-    for (int i = 0; i < ND4; i += 4) {
+    const int ns4 = ns / 4 * 4;
+    for (int i = 0; i < ns4; i += 4) {
       uint4 bb = *reinterpret_cast<const uint4 *>(b + i);
       step(t, a, bb.x);
       step(t, a, bb.y);
       step(t, a, bb.z);
       step(t, a, bb.w);
     }
-#pragma unroll
-    for (int i = ND4; i < ND; i++) step(t, a, b[i]);
+#pragma unroll 3
+    for (int i = ns4; i < ns; i++) step(t, a, b[i]);
     normalize(r, t);
   }
 
@@ -187,9 +202,11 @@ struct Mont {
 #pragma unroll
     for (int k = 0; k < K; k++) t[k] = 0;
     // one lane's K steps are unrolled; the loop over lanes is kept (unrolling
-    // it too only raises register pressure)
+    // it too only raises register pressure) and stops after the nl lanes
+    // that hold digits
+    const int nlr = (int)nl;
 #pragma unroll 1
-    for (int li = 0; li < L; li++) {
+    for (int li = 0; li < nlr; li++) {
       // fresh copies behind an empty asm each iteration, so the compiler
       // cannot hoist the digits' 64-bit extensions out of the loop (that
       // doubles the registers they take and spills).  Groups of two to eight
@@ -290,6 +307,7 @@ __device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, u
   const int lane = threadIdx.x;
   const int grp = lane / L;
   M mont(lane);
+  mont.nl = powm_active_lanes(L, K, D, args.mod.len);
   const int l = mont.g.l;
   const uint32_t local = tile * G + grp;
   const bool active = local < args.count;
@@ -498,6 +516,7 @@ __global__ void __launch_bounds__(64, 2) r2_square_kernel(R2Args args) {
   const int lane = threadIdx.x;
   const int grp = lane / L;
   M mont(lane);
+  mont.nl = powm_active_lanes(L, K, D, args.mod.len);
   const int l = mont.g.l;
   const uint32_t local = blockIdx.x * G + grp;
   const bool active = local < args.count;
@@ -700,7 +719,7 @@ hipError_t launch_montgomery_r2(int variant, uint32_t *r2, uint32_t r2_stride, c
                                 uint32_t m_stride, uint32_t count, uint32_t *ws, hipStream_t stream) {
   if (variant < 0 || variant >= kNumVariants || count == 0) return hipErrorInvalidValue;
   const PowmVariant &v = kVariants[variant].v;
-  const uint32_t rbits = variant_rbits(v);
+  const uint32_t rbits = variant_active_rbits(v, m_len);
   if (m_len <= kMaxRegDivisorLimbs) return launch_pow2_mod(r2, m_len, r2_stride, m, m_len, m_stride, count, 2 * rbits,
                                                            nullptr, stream);
   void (*kernel)(R2Args) = nullptr;

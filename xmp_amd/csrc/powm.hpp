@@ -45,8 +45,22 @@ int powm_variant_count();
 PowmVariant powm_variant(int i);
 // index of the compiled variant (L, K, D) or -1
 int powm_find_variant(int lanes, int digits, int radix);
-// smallest R exponent (bits) a variant gives
+// largest R exponent (bits) a variant gives: every lane's digits
 inline uint32_t variant_rbits(const PowmVariant &v) { return (uint32_t)(v.lanes * v.digits * v.radix); }
+// Lanes of a group that hold digits of a modulus of m_len limbs: the fewest
+// whose K*D-bit share reaches 32*m_len + 2 bits (R >= 4m), at most L.  The
+// Montgomery steps run over these lanes' digits only (the digits above them
+// are zero), so R = 2^(D*K*lanes): a 1024-bit modulus on 16 lanes of 3
+// radix-2^28 digits takes 13 lanes' 39 steps per product instead of 48.
+__host__ __device__ inline uint32_t powm_active_lanes(uint32_t L, uint32_t K, uint32_t D, uint32_t m_len) {
+  const uint32_t nd = (32u * m_len + 2u + D - 1u) / D;
+  const uint32_t nl = (nd + K - 1u) / K;
+  return nl < 1u ? 1u : (nl > L ? L : nl);
+}
+// the R exponent (bits) a variant runs with for moduli of m_len limbs
+inline uint32_t variant_active_rbits(const PowmVariant &v, uint32_t m_len) {
+  return (uint32_t)(v.digits * v.radix) * powm_active_lanes(v.lanes, v.digits, v.radix, m_len);
+}
 // table workspace in 32-bit words for `count` instances
 size_t powm_table_words(const PowmVariant &v, uint32_t count, uint32_t window_bits);
 // launch the exponentiation kernel for one variant

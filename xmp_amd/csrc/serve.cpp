@@ -499,7 +499,7 @@ xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const
       (e = xmpIntegersImport(handle, s->DPQ, s->h32, -1, 4, 0, 0, dpq.data(), 2 * nkeys)) ||
       (e = xmpIntegersImport(handle, s->CPQ, s->n32, -1, 4, 0, 0, cpq.data(), 2 * nkeys)))
     return fail(e);
-  // Layouts for batch sizes 64, 256, ... up to the capacity; with prepare,
+  // Layouts for batch sizes 64, 128, 256, ... up to the capacity; with prepare,
   // R^2 mod m for each distinct layout; returns the table words one slot
   // needs per kind and the device bytes one in-flight request holds.
   size_t tab_a = 0, tab_b = 0;
@@ -514,7 +514,7 @@ xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const
       // up to the largest batch size a layout is chosen for: the capacity
       // times the layout scale (at most the number of slots)
       const uint32_t top = cap * (config->layout_scale ? config->layout_scale : config->streams);
-      for (uint32_t c = 64; c < top; c *= 4) counts.push_back(c * mult);
+      for (uint32_t c = 64; c < top; c *= 2) counts.push_back(c * mult);
       counts.push_back(top * mult);
       for (uint32_t c : counts) {
         PowmChoice ch;

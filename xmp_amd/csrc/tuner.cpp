@@ -141,9 +141,61 @@ xmpError_t choose_powm(xmpHandle_t h, uint32_t precision, uint32_t exp_bits, uin
     if (alg == xmpAlgorithmDistributedMP) return v.lanes > 1;
     return true;
   };
-  // measured entries for this precision: the batch size nearest the request
-  // (log scale, same exponent size preferred), then the fastest allowed layout
-  // measured at that size
+  // Measured entries for this precision, at the exponent size measured nearest
+  // the request's.  Each layout's time for `count` instances is read from its
+  // times at the measured batch sizes either side, interpolated on a log-log
+  // scale (a batch of 8,192 between sizes 4,096 and 16,384 is priced from
+  // both, not from whichever is nearer); outside the measured sizes the
+  // nearest size stands in.  The fastest allowed layout wins.
+  {
+    uint32_t eb = 0;
+    double ebd = 1e30;
+    for (const TunedEntry &e : g_table) {
+      if (e.precision != precision || !e.exp_bits) continue;
+      double d = std::fabs(std::log((double)e.exp_bits) - std::log((double)std::max(exp_bits, 1u)));
+      if (d < ebd) {
+        ebd = d;
+        eb = e.exp_bits;
+      }
+    }
+    uint32_t lo = 0, hi = 0;   // largest measured size <= count, smallest >= count
+    for (const TunedEntry &e : g_table) {
+      if (e.precision != precision || e.exp_bits != eb || !eb) continue;
+      if (e.count <= count) lo = std::max(lo, e.count);
+      if (e.count >= count) hi = hi ? std::min(hi, e.count) : e.count;
+    }
+    if (!lo) lo = hi;
+    if (!hi) hi = lo;
+    const TunedEntry *pick = nullptr;
+    double pick_t = 1e300;
+    for (const TunedEntry &a : g_table) {
+      if (!lo || a.precision != precision || a.exp_bits != eb || a.count != lo || a.rate <= 0) continue;
+      int vi = powm_find_variant(a.lanes, a.digits, a.radix);
+      if (vi < 0 || !allowed(powm_variant(vi))) continue;
+      const TunedEntry *b = hi == lo ? &a : nullptr;
+      for (const TunedEntry &c : g_table)
+        if (!b && c.precision == precision && c.exp_bits == eb && c.count == hi && c.rate > 0 && c.lanes == a.lanes &&
+            c.digits == a.digits && c.radix == a.radix)
+          b = &c;
+      if (!b) continue;
+      const double ta = std::log((double)a.count / a.rate), tb = std::log((double)b->count / b->rate);
+      const double f = hi == lo ? 0.0
+                                : (std::log((double)count) - std::log((double)lo)) /
+                                      (std::log((double)hi) - std::log((double)lo));
+      const double t = ta + (tb - ta) * f;
+      if (t < pick_t) {
+        pick_t = t;
+        pick = f <= 0.5 ? &a : b;   // the window measured at the nearer size
+      }
+    }
+    if (pick) {
+      out->variant = powm_find_variant(pick->lanes, pick->digits, pick->radix);
+      out->window_bits = eb == exp_bits ? pick->window : best_window_bits(exp_bits);
+      return xmpErrorSuccess;
+    }
+  }
+  // entries without a rate: the batch size nearest the request (log scale,
+  // same exponent size preferred), then the fastest allowed layout there
   const TunedEntry *best = nullptr;
   double best_d = 1e30, best_rate = -1;
   for (const TunedEntry &e : g_table) {

@@ -7,8 +7,9 @@ Here, for each precision class and batch size, xmpTunerRun times every lane
 layout (lanes x digits x radix) and the window widths around the modelled best
 on random per-instance moduli; the table is written as JSON next to the
 library (xmp_amd/tune_gfx950.json), where the runtime loads it by default
-($XMP_TUNE_FILE overrides).  The choice for a call is the fastest measured
-layout at the nearest measured batch size.
+($XMP_TUNE_FILE overrides).  The choice for a call is the layout with the
+shortest time for its batch size, each layout's time interpolated (log-log)
+between the measured batch sizes either side.
 
 Usage (on the GPU):  python -m xmp_amd.tune [--out PATH] [--quick]
 """
@@ -30,9 +31,9 @@ PLAN = [
     (256, 256, [4096, 65536]),
     (512, 512, [4096, 65536, 262144]),
     (768, 768, [16384]),
-    (1024, 1024, [1024, 4096, 16384, 65536]),
+    (1024, 1024, [1024, 4096, 8192, 16384, 65536]),
     (1536, 1536, [16384]),
-    (2048, 2048, [1024, 4096, 16384, 65536]),
+    (2048, 2048, [1024, 4096, 8192, 16384, 65536]),
     (3072, 3072, [4096, 16384]),
     (4096, 4096, [4096, 16384]),
     (6144, 6144, [4096]),
