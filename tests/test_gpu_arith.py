@@ -189,6 +189,40 @@ def test_mul_full_size_sample(handle):
     assert np.array_equal(got[:, :64], a) and not got[:, 64:].any()
 
 
+@pytest.mark.parametrize("bits_a,bits_b,bits_p,square", [(1024, 1024, 2048, False), (2048, 1024, 3072, False),
+                                                         (2048, 2048, 2048, False), (2048, 2048, 4160, False),
+                                                         (1024, 1024, 2048, True), (2048, 2048, 2048, True),
+                                                         (2048, 2048, 4096, True)])
+def test_mul_large_batches(handle, bits_a, bits_b, bits_p, square):
+    """400,000 products on the product-scanning kernels: every 997th row and
+    the last 64 (the last, partial tile) against GMP, with truncated and
+    zero-extended outputs, squares, and a shuffled output index array with a
+    gathered first operand."""
+    n = 400000
+    a = inputs.uniform(61 + bits_a, n, bits_a // 32)
+    b = a if square else inputs.uniform(62 + bits_b, n, bits_b // 32)
+    A = handle.from_array(a)
+    B = A if square else handle.from_array(b)
+    P = handle.integers(bits_p, n)
+    handle.mul(P, A, B)
+    rows = np.concatenate([np.arange(0, n, 997), np.arange(n - 64, n)])
+    got = P.get()
+    assert_rows_equal(got[rows], gmp(O.GO_MUL, rows.size, bits_p // 32, a[rows], b[rows]), "mul, 400,000 products")
+    if square:
+        return
+    rng = np.random.default_rng(bits_p)
+    ia, ip = rng.permutation(n).astype(np.uint32), rng.permutation(n).astype(np.uint32)
+    handle.set_indices(0, ip)
+    handle.set_indices(1, ia)
+    try:
+        handle.mul(P, A, B)
+    finally:
+        handle.clear_indices()
+    got = P.get()
+    want = gmp(O.GO_MUL, rows.size, bits_p // 32, a[ia[rows]], b[rows])
+    assert_rows_equal(got[ip[rows]], want, "indexed mul, 400,000 products")
+
+
 def test_mul_closed_form_and_square(handle):
     """(2^k - 1)^2 = 2^2k - 2^(k+1) + 1, and mul(a, a) takes the square path."""
     n = 64

@@ -326,6 +326,7 @@ struct ColMul {
   uint32_t inst;             // the product's slot within a row
   uint32_t len;
   uint32_t dig[NC + 2];   // the product's digits; dig[NC + 1] = 0 pads the last partial limb
+  uint64_t acc;           // the running column sum
 
   // the product a_i b_j of column i + j (for a square only i < j)
   template <int I, int J>
@@ -363,7 +364,7 @@ struct ColMul {
   }
 
   template <int C>
-  __device__ __forceinline__ bool step(uint64_t &acc) {
+  __device__ __forceinline__ bool step() {
     constexpr int LO = C - NB + 1 > 0 ? C - NB + 1 : 0;
     constexpr int HI = C < NA - 1 ? C : NA - 1;
     acc = col<C>(acc, std::make_integer_sequence<int, HI - LO + 1>());
@@ -372,28 +373,31 @@ struct ColMul {
     return emit<C>();
   }
 
-  template <int... C>
-  __device__ __forceinline__ bool run(std::integer_sequence<int, C...>) {
-    uint64_t acc = 0;
-    // stops at the first column after which every output limb is written
-    const bool done = (step<C>(acc) || ...);
-    if (done) return true;
-    dig[NC] = (uint32_t)acc;   // the top digit (< 2^D: the product has NC+1 digits)
+  // columns OFF, OFF+1, ...; true once every output limb is written (the
+  // columns after that are skipped)
+  template <int OFF, int... C>
+  __device__ __forceinline__ bool cols(std::integer_sequence<int, C...>) {
+    return (step<OFF + C>() || ...);
+  }
+
+  // the top digit (< 2^D: the product has NC+1 digits) and the last limbs
+  __device__ __forceinline__ bool finish() {
+    dig[NC] = (uint32_t)acc;
     dig[NC + 1] = 0;
     return emit<NC>() || emit<NC + 1>();
   }
+
+  // every column; true if the output ended inside the product
+  __device__ __forceinline__ bool run() { return cols<0>(std::make_integer_sequence<int, NC>()) || finish(); }
 };
 
-// digits 0 .. N-1 (radix 2^D) of an integer of len 32-bit limbs, lane-private
-// (the lane's digits start at bit 0, so every shift is a constant); limbs at
-// or above len read as zero
-template <int N, int D>
-__device__ __forceinline__ void load_digits(uint32_t (&d)[N], const uint32_t *data, uint32_t stride, uint32_t inst,
-                                            uint32_t len) {
-  constexpr int NL = (D * N + 31) / 32;
-  uint32_t w[NL + 1];
-  // the row pointer is uniform (scalar) and the instance a 32-bit lane
-  // offset, so a load needs no vector address arithmetic
+// 32-bit limbs 0 .. NL-1 of an integer of len limbs at a lane's slot (limbs
+// at or above len read as zero).  The row pointer is uniform (scalar) and
+// the slot a 32-bit lane offset, so a load needs no vector address
+// arithmetic; the branches are uniform, so every load is in flight at once.
+template <int NL>
+__device__ __forceinline__ void load_limbs(uint32_t (&w)[NL], const uint32_t *data, uint32_t stride, uint32_t inst,
+                                           uint32_t len) {
   const uint32_t *row = data;
   const uint32_t off = inst * 4u;
 #pragma unroll
@@ -402,12 +406,30 @@ __device__ __forceinline__ void load_digits(uint32_t (&d)[N], const uint32_t *da
     if ((uint32_t)j < len) w[j] = ld_row(row, off);   // uniform branch
     row += stride;
   }
-  w[NL] = 0;
+}
+
+// digits 0 .. N-1 (radix 2^D) of the limbs w (the lane's digits start at
+// bit 0, so every shift is a constant)
+template <int N, int D, int NL>
+__device__ __forceinline__ void limbs_to_digits(uint32_t (&d)[N], const uint32_t (&w)[NL]) {
+  static_assert(NL == (D * N + 31) / 32, "limb count");
 #pragma unroll
   for (int k = 0; k < N; k++) {
     const int bit = D * k, j = bit >> 5, s = bit & 31;
-    d[k] = (s + D <= 32 ? (w[j] >> s) : __builtin_amdgcn_alignbit(w[j + 1], w[j], s)) & ((1u << D) - 1);
+    const uint32_t hi = j + 1 < NL ? w[j + 1] : 0u;
+    d[k] = (s + D <= 32 ? (w[j] >> s) : __builtin_amdgcn_alignbit(hi, w[j], s)) & ((1u << D) - 1);
   }
+}
+
+// digits 0 .. N-1 (radix 2^D) of an integer of len 32-bit limbs, lane-private;
+// limbs at or above len read as zero
+template <int N, int D>
+__device__ __forceinline__ void load_digits(uint32_t (&d)[N], const uint32_t *data, uint32_t stride, uint32_t inst,
+                                            uint32_t len) {
+  constexpr int NL = (D * N + 31) / 32;
+  uint32_t w[NL];
+  load_limbs<NL>(w, data, stride, inst, len);
+  limbs_to_digits<N, D, NL>(d, w);
 }
 
 template <int NA, int NB, int D, bool SQR>
@@ -434,8 +456,8 @@ __global__ void __launch_bounds__(64, 3) mul_cols_kernel(OutOperand p, Operand a
     load_digits<NB, D>(db, b.data, b.stride, bi, b.len);
   }
   using CM = ColMul<NA, NB, D, SQR>;
-  CM m{da, db, p.data, p.stride, pi, p.len, {}};
-  if (m.run(std::make_integer_sequence<int, CM::NC>())) return;
+  CM m{da, db, p.data, p.stride, pi, p.len, {}, 0};
+  if (m.run()) return;
   // past the product: zero limbs
   for (uint32_t w = (uint32_t)CM::NOUT; w < p.len; w++) p.data[(size_t)w * p.stride + pi] = 0u;
 }

@@ -201,6 +201,9 @@ struct Mont {
     uint64_t t[K];
 #pragma unroll
     for (int k = 0; k < K; k++) t[k] = 0;
+    uint32_t a2[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) a2[k] = kOwnDoubled ? a[k] << 1 : 0u;
     // one lane's K steps are unrolled; the loop over lanes is kept (unrolling
     // it too only raises register pressure) and stops after the nl lanes
     // that hold digits
@@ -212,18 +215,26 @@ struct Mont {
       // doubles the registers they take and spills).  Groups of two to eight
       // lanes with up to 38 digits fit their registers without it under the
       // ILP scheduler (no spill at all for 2x37, whose launch is then
-      // 1.2-1.8 % faster; 4x38 and 8x38 run 0.5 % faster); 2x55 keeps it.
-      constexpr bool kCopyBarrier = !((L == 2 || L == 4 || L == 8) && K <= 38);
+      // 1.2-1.8 % faster; 4x38 and 8x38 run 0.5 % faster); 2x55 keeps it,
+      // and layouts of up to 10 digits per lane have registers to spare.
+      constexpr bool kCopyBarrier = K > 10 && !((L == 2 || L == 4 || L == 8) && K <= 38);
       uint32_t ac[K];
 #pragma unroll
       for (int k = 0; k < K; k++) {
         ac[k] = a[k];
         if constexpr (kCopyBarrier) asm("" : "+v"(ac[k]));
       }
-      gsqr_steps(t, ac, b + li * K, std::make_integer_sequence<int, K>());
+      gsqr_steps(t, ac, a2, b + li * K, std::make_integer_sequence<int, K>());
     }
     normalize(r, t);
   }
+
+  // Layouts of up to 10 digits per lane keep their own digits doubled in K
+  // more registers, so a step multiplies 2a_j by b_i instead of doubling b_i
+  // first: K shifts per squaring instead of one per step (a step of 16x3 is
+  // 14 VALU instructions).  With more digits the step is long enough that
+  // the shift matters less than the registers.
+  static constexpr bool kOwnDoubled = K <= 10;
 
   static constexpr bool beats(int x, int y) {
     const int d = (y - x + K) % K;
@@ -231,38 +242,46 @@ struct Mont {
   }
 
   template <int... I>
-  __device__ __forceinline__ void gsqr_steps(uint64_t (&t)[K], const uint32_t (&a)[K], const uint32_t *bl,
-                                             std::integer_sequence<int, I...>) const {
-    (gsqr_step<I>(t, a, bl), ...);
+  __device__ __forceinline__ void gsqr_steps(uint64_t (&t)[K], const uint32_t (&a)[K], const uint32_t (&a2)[K],
+                                             const uint32_t *bl, std::integer_sequence<int, I...>) const {
+    (gsqr_step<I>(t, a, a2, bl), ...);
+  }
+
+  // 2 a_j b_i, from the doubled own digit or the doubled multiplier digit
+  __device__ __forceinline__ uint64_t twice(const uint32_t (&a)[K], const uint32_t (&a2)[K], int j, uint32_t bi,
+                                            uint32_t b2) const {
+    return kOwnDoubled ? (uint64_t)a2[j] * bi : (uint64_t)a[j] * b2;
   }
 
   template <int KI>
-  __device__ __forceinline__ void gsqr_step(uint64_t (&t)[K], const uint32_t (&a)[K], const uint32_t *bl) const {
+  __device__ __forceinline__ void gsqr_step(uint64_t (&t)[K], const uint32_t (&a)[K], const uint32_t (&a2)[K],
+                                            const uint32_t *bl) const {
     const uint32_t bi = bl[KI];
-    const uint32_t b2 = bi << 1;
+    const uint32_t b2 = kOwnDoubled ? 0u : bi << 1;
     uint64_t f = t[0];
     if constexpr (KI == 0) f += (uint64_t)a[0] * bi;
-    else if constexpr (beats(KI, 0)) f += (uint64_t)a[0] * b2;
+    else if constexpr (beats(KI, 0)) f += twice(a, a2, 0, bi, b2);
     const uint32_t q = g.bcast0(mul_lo_mad((uint32_t)f, np0)) & mask;
     f += (uint64_t)q * m[0];
     const uint32_t up = g.from_next_z((uint32_t)f) & mask;
-    gsqr_cols<KI>(t, a, q, bi, b2, std::make_integer_sequence<int, K - 1>());
+    gsqr_cols<KI>(t, a, a2, q, bi, b2, std::make_integer_sequence<int, K - 1>());
     t[K - 1] = up;
     t[0] += f >> D;
   }
 
   template <int KI, int... J>
-  __device__ __forceinline__ void gsqr_cols(uint64_t (&t)[K], const uint32_t (&a)[K], uint32_t q, uint32_t bi,
-                                            uint32_t b2, std::integer_sequence<int, J...>) const {
-    (gsqr_col<KI, J + 1>(t, a, q, bi, b2), ...);
+  __device__ __forceinline__ void gsqr_cols(uint64_t (&t)[K], const uint32_t (&a)[K], const uint32_t (&a2)[K],
+                                            uint32_t q, uint32_t bi, uint32_t b2,
+                                            std::integer_sequence<int, J...>) const {
+    (gsqr_col<KI, J + 1>(t, a, a2, q, bi, b2), ...);
   }
 
   template <int KI, int J>
-  __device__ __forceinline__ void gsqr_col(uint64_t (&t)[K], const uint32_t (&a)[K], uint32_t q, uint32_t bi,
-                                           uint32_t b2) const {
+  __device__ __forceinline__ void gsqr_col(uint64_t (&t)[K], const uint32_t (&a)[K], const uint32_t (&a2)[K],
+                                           uint32_t q, uint32_t bi, uint32_t b2) const {
     uint64_t x = t[J] + (uint64_t)q * m[J];
     if constexpr (J == KI) x += (uint64_t)a[J] * bi;
-    else if constexpr (beats(KI, J)) x += (uint64_t)a[J] * b2;
+    else if constexpr (beats(KI, J)) x += twice(a, a2, J, bi, b2);
     t[J - 1] = x;
   }
 };
