@@ -58,11 +58,45 @@ def test_server_alone_and_in_mix(handle):
         srv.close()
 
 
+@pytest.mark.parametrize("bits", [1024, 2048, 3072, 4096])
+def test_server_crt_recombination(handle, bits):
+    """Up to 4096-bit keys the kind-1 recombination (mp*cp + mq*cq) mod n is
+    one Montgomery reduction against the resident cp*R mod n and cq*R mod n.
+    Exact against the oracle with the keys' CRT coefficients (keys 0-2) and
+    with arbitrary ones, for which the result is still that linear
+    combination mod n: all ones (above n), zero and n - 1, and random words."""
+    nkeys, nreq = 6, 64
+    keys = make_keys(nkeys, bits, 900 + bits)
+    n32 = bits // 32
+    keys["cp"][3] = 0xFFFFFFFF
+    keys["cq"][3] = 0xFFFFFFFF
+    keys["cp"][4] = 0
+    keys["cq"][4] = keys["n"][4]
+    keys["cq"][4, 0] ^= 1                      # n - 1 (n is odd)
+    keys["cp"][5] = inputs.uniform(bits + 7, 1, n32)[0]
+    keys["cq"][5] = inputs.uniform(bits + 8, 1, n32)[0]
+    rng = np.random.default_rng(bits + 3)
+    kinds = (rng.random(nreq) < 0.8).astype(np.uint32)
+    key_idx = (np.arange(nreq) % nkeys).astype(np.uint32)
+    bases = inputs.uniform(bits + 5, nreq, n32)
+    nmat = keys["n"][key_idx]
+    bases[:, -1] = (bases[:, -1].astype(np.uint64) % nmat[:, -1].astype(np.uint64)).astype(np.uint32)
+    exps = inputs.uniform(bits + 6, nreq, n32)
+    want = O.serving_expected(keys, kinds, key_idx, bases, exps, threads=THREADS)
+    srv = Server(handle, keys, bits, streams=2, max_batch=16)
+    try:
+        res, lat, wall = srv.run_poisson(1e6, kinds, key_idx, bases, exps, seed=11)
+    finally:
+        srv.close()
+    assert_rows_equal(res, want, "%d-bit CRT recombination" % bits)
+
+
 @pytest.mark.parametrize("bits,nkeys,nreq", [(8192, 4, 16), (16384, 2, 6)])
 def test_server_wide_keys(handle, bits, nkeys, nreq):
-    """Keys above 4096 bits: the CRT chain's final division by n (256 limbs at
-    8192 bits) and, at 16384 bits, also the division by p and q (256 limbs)
-    go past the register division and use the slot's division workspace."""
+    """Keys above 4096 bits keep the multiply, add and division chain: its
+    final division by n (256 limbs at 8192 bits) and, at 16384 bits, also the
+    division by p and q (256 limbs) go past the register division and use
+    the slot's division workspace."""
     keys = make_keys(nkeys, bits, 500 + bits)
     rng = np.random.default_rng(bits)
     kinds = np.array([i % 2 for i in range(nreq)], np.uint32)

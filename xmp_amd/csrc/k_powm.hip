@@ -37,6 +37,7 @@
 // The exponent is scanned over all of its precision's bits with a fixed
 // window (the reference's constant-time schedule, powm.h:94-98), and the
 // result is reduced to the canonical value < m (GMP semantics).
+#include <cmath>
 #include <utility>
 
 #include "devmp.hpp"
@@ -101,6 +102,23 @@ struct Mont {
     const uint32_t up = g.from_next_z((uint32_t)f) & mask;
 #pragma unroll
     for (int j = 1; j < K; j++) t[j - 1] = t[j] + (uint64_t)a[j] * bi + (uint64_t)q * m[j];
+    t[K - 1] = up;
+    t[0] += f >> D;
+  }
+
+  // One CIOS step of a sum of two products: t = (t + a*b_i + c*d_i + q*m) / 2^D.
+  // A position collects three products of < 2^(2D) per step, so `steps`
+  // such steps need 3*steps*2^(2D) plus a carry below 2^64 (checked by the
+  // launcher).
+  __device__ __forceinline__ void step2(uint64_t (&t)[K], const uint32_t (&a)[K], uint32_t bi,
+                                        const uint32_t (&c)[K], uint32_t di) const {
+    uint64_t f = t[0] + (uint64_t)a[0] * bi + (uint64_t)c[0] * di;
+    const uint32_t q = g.bcast0(mul_lo_mad((uint32_t)f, np0)) & mask;
+    f += (uint64_t)q * m[0];
+    const uint32_t up = g.from_next_z((uint32_t)f) & mask;
+#pragma unroll
+    for (int j = 1; j < K; j++)
+      t[j - 1] = t[j] + (uint64_t)a[j] * bi + (uint64_t)c[j] * di + (uint64_t)q * m[j];
     t[K - 1] = up;
     t[0] += f >> D;
   }
@@ -600,6 +618,89 @@ const R2Entry kR2Variants[] = {
 };
 #undef XMP_R2
 
+// ---- CRT recombination ---------------------------------------------------
+// out = (mp*cp + mq*cq) mod n for each request (the server's kind-1 tail,
+// reference samples/crt_rsa/rsa.cc:205-219) as one Montgomery reduction:
+// with cp2 = cp*R mod n and cq2 = cq*R mod n resident (R = 2^(D*steps)),
+// REDC(mp*cp2 + mq*cq2) = (mp*cp + mq*cq) mod n up to one subtraction of n,
+// since the sum is below 2^(h+1) n for h-bit mp and mq and R >= 2^(h+1).
+// Each step takes one digit of mp and of mq (staged in LDS) against the
+// lane group's digits of cp2 and cq2, with one quotient digit for both.
+template <int L, int K, int D>
+__global__ void __launch_bounds__(64, 2) crt_combine_kernel(CrtArgs args) {
+  using M = Mont<L, K, D>;
+  constexpr int ND = M::ND;
+  constexpr int NDP = (ND + 3) / 4 * 4;
+  constexpr int G = 64 / L;
+  __shared__ __attribute__((aligned(16))) uint32_t buf[2 * G * NDP];
+  const int lane = threadIdx.x;
+  const int grp = lane / L;
+  M mont(lane);
+  const int l = mont.g.l;
+  const uint32_t local = blockIdx.x * G + grp;
+  const bool active = local < args.count;
+  const uint32_t i = active ? local : args.count - 1;   // clamp: inactive lanes recompute a valid request
+  uint32_t *b = buf + grp * NDP;         // mp's digits, then the result's
+  uint32_t *d = buf + (G + grp) * NDP;   // mq's digits, then n's
+  const uint32_t ni = in_index(args.n, i), pi = in_index(args.cp2, i), qi = in_index(args.cq2, i);
+  const uint32_t xi = in_index(args.mp, i), yi = in_index(args.mq, i);
+  uint32_t a[K], c[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const uint32_t dk = (uint32_t)(l * K + k);
+    mont.m[k] = limb_digit_guarded<D>(args.n.data, args.n.stride, ni, args.n.len, dk);
+    a[k] = limb_digit_guarded<D>(args.cp2.data, args.cp2.stride, pi, args.cp2.len, dk);
+    c[k] = limb_digit_guarded<D>(args.cq2.data, args.cq2.stride, qi, args.cq2.len, dk);
+    b[dk] = limb_digit_guarded<D>(args.mp.data, args.mp.stride, xi, args.mp.len, dk);
+    d[dk] = limb_digit_guarded<D>(args.mq.data, args.mq.stride, yi, args.mq.len, dk);
+  }
+  {
+    // np0 = -n^-1 mod 2^D (Newton)
+    const uint32_t n0 = ld_limb(args.n, ni, 0);
+    uint32_t inv = n0;
+#pragma unroll
+    for (int it = 0; it < 4; it++) inv = inv * (2u - n0 * inv);
+    mont.np0 = (0u - inv) & M::MASK;
+  }
+  wave_sync();
+  uint64_t t[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) t[k] = 0;
+  for (uint32_t s = 0; s < args.steps; s++) mont.step2(t, a, b[s], c, d[s]);
+  uint32_t r[K];
+  mont.normalize(r, t);
+  wave_sync();
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    b[l * K + k] = r[k];
+    d[l * K + k] = mont.m[k];
+  }
+  wave_sync();
+  if (l == 0) {
+    // r < 2n: r - n into d (digit-serial borrow), kept where it did not borrow out
+    uint32_t br = 0;
+    for (int j = 0; j < ND; j++) {
+      const uint32_t x = b[j] + (M::MASK + 1u) - d[j] - br;
+      d[j] = x & M::MASK;
+      br = 1u - (x >> D);
+    }
+    if (!br)
+      for (int j = 0; j < ND; j++) b[j] = d[j];
+  }
+  wave_sync();
+  if (active) store_limbs<L, D>(b, ND, args.out.data, args.out.stride, out_index(args.out, i), args.out.len, l);
+}
+
+struct CrtEntry {
+  PowmVariant v;
+  void (*kernel)(CrtArgs);
+};
+#define XMP_CRT(L, K, D) {{L, K, D}, crt_combine_kernel<L, K, D>}
+// moduli n below 2^1064, 2^2128 and 2^4104; 19 digits per lane keep the three
+// operands and the accumulator in registers (37 spill)
+const CrtEntry kCrtVariants[] = {XMP_CRT(2, 19, 28), XMP_CRT(4, 19, 28), XMP_CRT(8, 19, 27)};
+#undef XMP_CRT
+
 // ---- persistent mixed-size kernel --------------------------------------
 // Every wave pulls (class, tile) pairs from a device queue ordered by
 // decreasing cost and runs the tile with its class's layout, so tiles of all
@@ -763,6 +864,36 @@ hipError_t launch_montgomery_r2(int variant, uint32_t *r2, uint32_t r2_stride, c
   hipLaunchKernelGGL(kernel, dim3(div_round_up(count, 64u / (uint32_t)v.lanes)), dim3(64), 0, stream, a);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   return launch_r2_canonical(r2, r2_stride, ws, ws_stride, m, m_len, m_stride, count, stream);
+}
+
+// The recombination layout for n-bit moduli and h-bit residues: the
+// smallest whose digits hold n (R >= 4n is not needed here, only n < 2^(D*ND)
+// with room for 2n) and whose lazy sums bear 3*steps products; steps = the
+// digits of R >= 2^(h+1).  Returns false when none fits.
+bool crt_combine_plan(uint32_t n_bits, uint32_t h_bits, int *which, uint32_t *steps) {
+  for (int w = 0; w < (int)(sizeof(kCrtVariants) / sizeof(kCrtVariants[0])); w++) {
+    const PowmVariant &v = kCrtVariants[w].v;
+    const uint32_t nd = (uint32_t)(v.lanes * v.digits);
+    const uint32_t ns = (h_bits + 1u + (uint32_t)v.radix - 1u) / (uint32_t)v.radix;
+    const double bound = 3.0 * ns * std::ldexp(1.0, 2 * v.radix) + std::ldexp(1.0, 64 - v.radix);
+    if ((uint32_t)v.radix * nd >= n_bits + 1 && ns <= nd && bound < std::ldexp(1.0, 64)) {
+      *which = w;
+      *steps = ns;
+      return true;
+    }
+  }
+  return false;
+}
+
+uint32_t crt_combine_radix(int which) { return (uint32_t)kCrtVariants[which].v.radix; }
+
+hipError_t launch_crt_combine(int which, const CrtArgs &args, hipStream_t stream) {
+  if (which < 0 || which >= (int)(sizeof(kCrtVariants) / sizeof(kCrtVariants[0])) || args.count == 0)
+    return hipErrorInvalidValue;
+  const uint32_t per_block = 64u / (uint32_t)kCrtVariants[which].v.lanes;
+  hipLaunchKernelGGL(kCrtVariants[which].kernel, dim3(div_round_up(args.count, per_block)), dim3(64), 0, stream,
+                     args);
+  return hipGetLastError();
 }
 
 }  // namespace xmp

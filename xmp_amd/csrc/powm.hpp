@@ -109,6 +109,20 @@ struct FixedWorkspace {
 xmpError_t powm_run(xmpHandle_t handle, xmpIntegers_t out, xmpIntegers_t a, xmpIntegers_t exp, xmpIntegers_t mod,
                     uint32_t count, const FixedWorkspace *ws);
 
+// CRT recombination out = (mp*cp + mq*cq) mod n, as REDC(mp*cp2 + mq*cq2)
+// with cp2 = cp*R mod n, cq2 = cq*R mod n and R = 2^(radix*steps).
+struct CrtArgs {
+  OutOperand out;        // n-bit results
+  Operand mp, mq;        // residues below 2^h_bits
+  Operand cp2, cq2, n;   // n-bit, gathered by key
+  uint32_t steps;
+  uint32_t count;
+};
+// the recombination layout and step count for n-bit moduli and h-bit residues
+bool crt_combine_plan(uint32_t n_bits, uint32_t h_bits, int *which, uint32_t *steps);
+uint32_t crt_combine_radix(int which);
+hipError_t launch_crt_combine(int which, const CrtArgs &args, hipStream_t stream);
+
 // Persistent mixed-size launch: tiles (class, tile) in queue order.
 struct BatchArgs {
   const PowmArgs *classes;        // per class (device array)

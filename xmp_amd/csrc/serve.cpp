@@ -80,6 +80,12 @@ struct _xmpServer_t {
   uint32_t bits = 0, n32 = 0, h32 = 0, nkeys = 0;
   // resident keys
   xmpIntegers_t N = nullptr, PQ = nullptr, DPQ = nullptr, CPQ = nullptr;
+  // the kind-1 recombination as one Montgomery reduction (crt >= 0: its
+  // layout; cp*R mod n and cq*R mod n resident, R = 2^(radix*crt_steps)),
+  // else the multiply, add and division chain
+  xmpIntegers_t CP2 = nullptr, CQ2 = nullptr;
+  int crt = -1;
+  uint32_t crt_steps = 0;
   // Montgomery layouts per request kind, chosen per batch by its size: the
   // tuner's choice at a few batch sizes up to the slot capacity, each with
   // its own R^2 mod m (R depends on the layout)
@@ -269,14 +275,28 @@ hipError_t launch_batch(xmpServer_t s, _xmpServer_t::Slot &sl, uint32_t scale) {
     if ((e = launch_powm_variant(ly.variant, a, sl.stream_b)) != hipSuccess ||
         (e = order(sl.stream_b, sb, sl.pow_b)))
       return e;
-    // sp = mp*cp, sq = mq*cq; r = (sp + sq) mod n
-    if ((e = launch_mul(outop(sl.b_prod), op(s->CPQ, d_keyPQ, 2 * nb), op(sl.b_pow), 2 * nb, sb)) != hipSuccess)
-      return e;
-    if ((e = launch_add(outop(sl.b_sum), op(sl.b_prod, d_lo, nb), op(sl.b_prod, d_hi, nb), nb, sb)) != hipSuccess)
-      return e;
-    if ((e = launch_divmod_ws(none, outop(sl.b_out), op(sl.b_sum), op(s->N, d_keyB, nb), nb, sl.div_ws, sb)) !=
-        hipSuccess)
-      return e;
+    if (s->crt >= 0) {
+      // r = (mp*cp + mq*cq) mod n in one launch
+      CrtArgs ca;
+      ca.out = outop(sl.b_out);
+      ca.mp = op(sl.b_pow, d_lo, nb);
+      ca.mq = op(sl.b_pow, d_hi, nb);
+      ca.cp2 = op(s->CP2, d_keyB, nb);
+      ca.cq2 = op(s->CQ2, d_keyB, nb);
+      ca.n = op(s->N, d_keyB, nb);
+      ca.steps = s->crt_steps;
+      ca.count = nb;
+      if ((e = launch_crt_combine(s->crt, ca, sb)) != hipSuccess) return e;
+    } else {
+      // sp = mp*cp, sq = mq*cq; r = (sp + sq) mod n
+      if ((e = launch_mul(outop(sl.b_prod), op(s->CPQ, d_keyPQ, 2 * nb), op(sl.b_pow), 2 * nb, sb)) != hipSuccess)
+        return e;
+      if ((e = launch_add(outop(sl.b_sum), op(sl.b_prod, d_lo, nb), op(sl.b_prod, d_hi, nb), nb, sb)) != hipSuccess)
+        return e;
+      if ((e = launch_divmod_ws(none, outop(sl.b_out), op(sl.b_sum), op(s->N, d_keyB, nb), nb, sl.div_ws, sb)) !=
+          hipSuccess)
+        return e;
+    }
     if ((e = launch_export((uint8_t *)(sl.g_out + words), sl.b_out.data, sl.b_out.stride, n32, nb, n32, -1, -1, 0, 4,
                            nullptr, 0, sb)) != hipSuccess)
       return e;
@@ -441,7 +461,7 @@ void free_server(xmpServer_t s) {
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
     if (sl.stream_b) (void)hipStreamDestroy(sl.stream_b);
   }
-  for (xmpIntegers_t x : {s->N, s->PQ, s->DPQ, s->CPQ})
+  for (xmpIntegers_t x : {s->N, s->PQ, s->DPQ, s->CPQ, s->CP2, s->CQ2})
     if (x) xmpIntegersDestroy(h, x);
   for (auto *set : {&s->lay_n, &s->lay_pq}) {
     std::vector<uint32_t *> freed;
@@ -499,6 +519,45 @@ xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const
       (e = xmpIntegersImport(handle, s->DPQ, s->h32, -1, 4, 0, 0, dpq.data(), 2 * nkeys)) ||
       (e = xmpIntegersImport(handle, s->CPQ, s->n32, -1, 4, 0, 0, cpq.data(), 2 * nkeys)))
     return fail(e);
+  // The kind-1 recombination as one Montgomery reduction when a layout holds
+  // n and every n is odd: cp*R mod n and cq*R mod n (a shift by the R
+  // exponent, then one division by n) stay resident.
+  {
+    bool odd = true;
+    for (uint32_t k = 0; k < nkeys; k++) odd = odd && (n[(size_t)k * s->n32] & 1u);
+    int which = -1;
+    uint32_t steps = 0;
+    if (odd && crt_combine_plan(bits, bits / 2, &which, &steps)) {
+      const int32_t sh = (int32_t)(crt_combine_radix(which) * steps);
+      const uint32_t tbits = bits + ((uint32_t)sh + 31u) / 32u * 32u;
+      xmpIntegers_t T = nullptr;
+      int32_t *d_sh = (int32_t *)handle->da(sizeof(int32_t));
+      if (!d_sh) return fail(xmpErrorInvalidCudaMalloc);
+      auto done = [&](xmpError_t err) {
+        if (T) xmpIntegersDestroy(handle, T);
+        handle->df(d_sh);
+        return err;
+      };
+      if ((e = xmpIntegersCreate(handle, &s->CP2, bits, nkeys)) || (e = xmpIntegersCreate(handle, &s->CQ2, bits, nkeys)) ||
+          (e = xmpIntegersCreate(handle, &T, tbits, 2 * nkeys)))
+        return fail(done(e));
+      OutOperand none{nullptr, 0, 0, nullptr};
+      const Operand t_p{T->limbs, T->nlimbs, T->stride, nkeys, nullptr, 0};
+      const Operand t_q{T->limbs + nkeys, T->nlimbs, T->stride, nkeys, nullptr, 0};
+      if (hipMemcpy(d_sh, &sh, sizeof sh, hipMemcpyHostToDevice) != hipSuccess ||
+          launch_shift(OutOperand{T->limbs, T->nlimbs, T->stride, nullptr}, op(s->CPQ), d_sh, 1, 2 * nkeys,
+                       handle->stream) != hipSuccess ||
+          launch_divmod_ws(none, OutOperand{s->CP2->limbs, s->CP2->nlimbs, s->CP2->stride, nullptr}, t_p, op(s->N),
+                           nkeys, nullptr, handle->stream) != hipSuccess ||
+          launch_divmod_ws(none, OutOperand{s->CQ2->limbs, s->CQ2->nlimbs, s->CQ2->stride, nullptr}, t_q, op(s->N),
+                           nkeys, nullptr, handle->stream) != hipSuccess ||
+          hipStreamSynchronize(handle->stream) != hipSuccess)
+        return fail(done(xmpErrorCuda));
+      done(xmpErrorSuccess);
+      s->crt = which;
+      s->crt_steps = steps;
+    }
+  }
   // Layouts for batch sizes 64, 128, 256, ... up to the capacity; with prepare,
   // R^2 mod m for each distinct layout; returns the table words one slot
   // needs per kind and the device bytes one in-flight request holds.
@@ -607,8 +666,11 @@ xmpError_t XMPAPI xmpServerCreate(xmpHandle_t handle, xmpServer_t *server, const
     if (alloc_array(handle, sl.a_base, s->n32, cap) || alloc_array(handle, sl.a_exp, s->n32, cap) ||
         alloc_array(handle, sl.a_out, s->n32, cap) || alloc_array(handle, sl.b_c, s->n32, cap) ||
         alloc_array(handle, sl.b_cm, s->h32, 2 * cap) || alloc_array(handle, sl.b_pow, s->h32, 2 * cap) ||
-        alloc_array(handle, sl.b_prod, s->n32 + s->h32, 2 * cap) ||
-        alloc_array(handle, sl.b_sum, s->n32 + s->h32 + 1, cap) || alloc_array(handle, sl.b_out, s->n32, cap))
+        alloc_array(handle, sl.b_out, s->n32, cap))
+      return fail(xmpErrorInvalidCudaMalloc);
+    // the products and their sum only for the multiply, add and division chain
+    if (s->crt < 0 && (alloc_array(handle, sl.b_prod, s->n32 + s->h32, 2 * cap) ||
+                       alloc_array(handle, sl.b_sum, s->n32 + s->h32 + 1, cap)))
       return fail(xmpErrorInvalidCudaMalloc);
     s->free_slots.push_back(k);
   }
