@@ -11,8 +11,9 @@
 // (nearly) full rate but every carry instruction costs as much as the
 // multiply:
 //
-//  * operands are held as radix-2^D digits (D = 28; 27 above 127 digits; 26
-//    above 511)
+//  * operands are held as radix-2^D digits (D = 28; the 4x37 and 8x37 groups
+//    run carry passes inside a product to keep D = 28 above 127 digits; the
+//    other wide layouts use D = 27 up to 511 digits and D = 26 up to 2047)
 //    and the CIOS accumulator as 64-bit "lazy" sums: each partial product is a
 //    single v_mad_u64_u32 into its 64-bit accumulator and no carry is
 //    propagated inside the multiplication (2*ND products of < 2^(2D) each fit
@@ -146,6 +147,8 @@ struct Mont {
       step(t, a, bb.y);
       step(t, a, bb.z);
       step(t, a, bb.w);
+      if constexpr (kCarryPasses)
+        if ((i + 4) % 64 == 0) carry_pass(t);
     }
 #pragma unroll 3
     for (int i = ns4; i < ns; i++) step(t, a, b[i]);
@@ -154,6 +157,26 @@ struct Mont {
 
   __device__ __forceinline__ void normalize(uint32_t (&r)[K], const uint64_t (&t)[K]) const {
     normalize_group<L, K, D>(r, t, g);
+  }
+
+  // Radix-2^28 layouts of more than 127 digits (4x37, 8x37) would overflow
+  // the 64-bit lazy sums over a whole product (2*ND products of < 2^56 per
+  // position), so they run carry passes within it: every position keeps its
+  // low D bits and hands the rest to the next position (the previous lane's
+  // top carry crosses by one DPP move; the group's top position has none,
+  // since the running value stays below 3m < R).  The value is unchanged and
+  // every sum drops below 2^D + 2^(64-D).  After that a position can take 85
+  // steps of up to three products' worth (a squaring step adds a doubled
+  // product and q*m_j: below 3*2^56), so a pass follows every 64 steps of a
+  // product and every floor(85/K) lanes of a squaring.
+  static constexpr bool kCarryPasses = D == 28 && ND > 127;
+  static constexpr int kPassLanes = 85 / K > 0 ? 85 / K : 1;
+
+  __device__ __forceinline__ void carry_pass(uint64_t (&t)[K]) const {
+    const uint64_t below = g.from_prev(t[K - 1] >> D);
+#pragma unroll
+    for (int k = K - 1; k >= 1; k--) t[k] = (t[k] & MASK) + (t[k - 1] >> D);
+    t[0] = (t[0] & MASK) + below;
   }
 
   // r = a^2 / R (mod m) for one-lane instances: the CIOS steps with the
@@ -243,6 +266,8 @@ struct Mont {
         if constexpr (kCopyBarrier) asm("" : "+v"(ac[k]));
       }
       gsqr_steps(t, ac, a2, b + li * K, std::make_integer_sequence<int, K>());
+      if constexpr (kCarryPasses)
+        if ((li + 1) % kPassLanes == 0 && li + 1 < nlr) carry_pass(t);
     }
     normalize(r, t);
   }
@@ -523,6 +548,9 @@ const VariantEntry kVariants[] = {
     XMP_LATENCY_VARIANT(8, 10, 28),  // 2240 : 2048-bit on eight lanes
     XMP_LATENCY_VARIANT(16, 5, 28),  // 2240 : 2048-bit on sixteen lanes
     XMP_LATENCY_VARIANT(16, 3, 28),  // 1344 : 1024-bit on sixteen lanes
+    // radix 2^28 above 127 digits, with carry passes inside each product
+    XMP_VARIANT(4, 37, 28),  // 4144 : 4096-bit
+    XMP_VARIANT(8, 37, 28),  // 8288 : 8192-bit
 };
 #undef XMP_VARIANT
 #undef XMP_LATENCY_VARIANT
@@ -615,6 +643,7 @@ struct R2Entry {
 // kMaxRegDivisorLimbs limbs)
 const R2Entry kR2Variants[] = {
     XMP_R2(8, 29, 27), XMP_R2(8, 38, 27), XMP_R2(16, 19, 27), XMP_R2(32, 20, 26), XMP_R2(64, 20, 26),
+    XMP_R2(8, 37, 28),
 };
 #undef XMP_R2
 
@@ -734,11 +763,13 @@ XMP_RUN(15, 1, 5, 28)
 XMP_RUN(16, 1, 10, 28)
 XMP_RUN(17, 32, 20, 26)
 XMP_RUN(18, 64, 20, 26)
+XMP_RUN(24, 4, 37, 28)
+XMP_RUN(25, 8, 37, 28)
 #undef XMP_RUN
 // Variant 9 (2x55, which spills as a called function) and the low-latency
 // layouts 19..23 are not part of the persistent kernel.
-static_assert(kNumVariants == 24, "run_variant<> must list every batched variant in kVariants order");
-constexpr bool batched_variant(int i) { return i >= 0 && i < 19 && i != 9; }
+static_assert(kNumVariants == 26, "run_variant<> must list every batched variant in kVariants order");
+constexpr bool batched_variant(int i) { return i >= 0 && (i < 19 || i == 24 || i == 25) && i != 9; }
 
 __global__ void __launch_bounds__(64, 2) powm_batch_kernel(BatchArgs ba) {
   extern __shared__ __attribute__((aligned(16))) uint32_t dyn_lds[];
@@ -769,6 +800,8 @@ __global__ void __launch_bounds__(64, 2) powm_batch_kernel(BatchArgs ba) {
       case 16: run_variant<16>(args, tile, dyn_lds); break;
       case 17: run_variant<17>(args, tile, dyn_lds); break;
       case 18: run_variant<18>(args, tile, dyn_lds); break;
+      case 24: run_variant<24>(args, tile, dyn_lds); break;
+      case 25: run_variant<25>(args, tile, dyn_lds); break;
       default: break;
     }
     wave_sync();
@@ -791,6 +824,7 @@ int powm_find_variant(int lanes, int digits, int radix) {
 }
 
 // lazy accumulators: 2*ND products of < 2^(2D) plus a carry must fit in 64 bits
+// (without carry passes)
 constexpr uint32_t max_digits(int radix) { return radix == 28 ? 127u : radix == 27 ? 511u : radix == 26 ? 2047u : 0u; }
 static_assert(2.0 * 127 * 268435455.0 * 268435455.0 + 68719476736.0 < 18446744073709551616.0, "radix 28 bound");
 static_assert(2.0 * 511 * 134217727.0 * 134217727.0 + 137438953472.0 < 18446744073709551616.0, "radix 27 bound");
@@ -798,7 +832,9 @@ static_assert(2.0 * 2047 * 67108863.0 * 67108863.0 + 274877906944.0 < 1844674407
 
 bool variant_fits(const PowmVariant &v, uint32_t precision) {
   const uint32_t nd = (uint32_t)(v.lanes * v.digits);
-  return nd <= max_digits(v.radix) && variant_rbits(v) >= precision + 2;
+  // radix-2^28 groups of more than 127 digits run carry passes (Mont::kCarryPasses)
+  const bool passes = v.radix == 28 && v.lanes > 1;
+  return (nd <= max_digits(v.radix) || passes) && variant_rbits(v) >= precision + 2;
 }
 
 size_t powm_table_words(const PowmVariant &v, uint32_t count, uint32_t window_bits) {
