@@ -126,8 +126,8 @@ xmpError_t XMPAPI xmpServerRunPoisson(xmpServer_t server, double rate_per_s, uin
                                       const uint32_t *exps, uint32_t *results, double *latency_us, double *wall_s);
 
 /* ---------------- tuner ---------------- */
-/* Measure every compiled layout that fits (and is at most about twice the
- * precision's width), with the window widths next to the modelled best, for
+/* Measure every compiled layout that fits (and is at most about three times
+ * the precision's width), with the window widths next to the modelled best, for
  * powm at `precision` bits with `count` instances on this device;
  * write one JSON object per measurement, comma separated, to `json`
  * (capacity `json_bytes`).  Returns the number of measurements in *measured. */

@@ -82,10 +82,11 @@ def test_every_layout(handle, layout):
         _restore()
 
 
-@pytest.mark.parametrize("layouts", [((4, 37, 28), (8, 37, 28)), ((4, 38, 27), (8, 38, 27))])
+@pytest.mark.parametrize("layouts", [((4, 37, 28), (8, 38, 27)), ((4, 38, 27), (16, 19, 27))])
 def test_batch_on_wide_layouts(handle, layouts):
-    """The persistent batch with a 4096-bit and an 8192-bit class on the
-    radix-2^28 groups with carry passes (and on their radix-2^27 siblings)."""
+    """The persistent batch with a 4096-bit and an 8192-bit class: the 4x37
+    radix-2^28 group with carry passes beside 8x38, and the radix-2^27
+    layouts (8x37 is not a batch layout)."""
     sizes, counts = (4096, 8192), (21, 9)
     try:
         _load([_entry(lay, bits, c) for lay, bits, c in zip(layouts, sizes, counts)])

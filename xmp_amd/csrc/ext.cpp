@@ -255,50 +255,66 @@ xmpError_t XMPAPI xmpTunerRun(xmpHandle_t handle, uint32_t precision, uint32_t e
     if (variant_rbits(v) > 3 * precision + 128) continue;
     uint32_t *r2 = nullptr, r2s = 0;
     if ((e = montgomery_prepare(handle, M, vi, &r2, &r2s)) != xmpErrorSuccess) break;
-    for (uint32_t w = (w0 > 2 ? w0 - 1 : 1); w <= w0 + 1 && w <= 8; w++) {
-      size_t tw = powm_table_words(v, round_up(count, kStrideAlign), w);
-      uint32_t *table = (uint32_t *)handle->da(tw * sizeof(uint32_t));
-      if (!table) continue;
-      PowmArgs args;
-      args.out = OutOperand{O->limbs, O->nlimbs, O->stride, nullptr};
-      args.base = Operand{B->limbs, B->nlimbs, B->stride, B->count, nullptr, 0};
-      args.exp = Operand{E->limbs, E->nlimbs, E->stride, E->count, nullptr, 0};
-      args.mod = Operand{M->limbs, M->nlimbs, M->stride, M->count, nullptr, 0};
-      args.r2 = r2;
-      args.r2_stride = r2s;
-      args.table = table;
-      args.window_bits = w;
-      args.start = 0;
-      args.count = count;
-      // one warm-up launch, then the median of three timed launches (clocks
-      // and neighbours vary from launch to launch)
-      float times[3] = {0, 0, 0};
-      for (int rep = 0; rep < 4; rep++) {
+    // the window widths around the cost-optimal one share one table (sized
+    // for the widest) and are timed in turns: a warm-up round, then four
+    // rounds of one launch per width, so that clock drift over the sweep
+    // falls on every width alike; each width keeps its median
+    const uint32_t wlo = w0 > 2 ? w0 - 1 : 1, whi = std::min(w0 + 1, 8u);
+    const uint32_t nw = whi - wlo + 1;
+    uint32_t *table = (uint32_t *)handle->da(powm_table_words(v, round_up(count, kStrideAlign), whi) *
+                                             sizeof(uint32_t));
+    if (!table) {
+      handle->df(r2);
+      continue;
+    }
+    PowmArgs args;
+    args.out = OutOperand{O->limbs, O->nlimbs, O->stride, nullptr};
+    args.base = Operand{B->limbs, B->nlimbs, B->stride, B->count, nullptr, 0};
+    args.exp = Operand{E->limbs, E->nlimbs, E->stride, E->count, nullptr, 0};
+    args.mod = Operand{M->limbs, M->nlimbs, M->stride, M->count, nullptr, 0};
+    args.r2 = r2;
+    args.r2_stride = r2s;
+    args.table = table;
+    args.start = 0;
+    args.count = count;
+    constexpr int kRounds = 4;
+    std::vector<float> times((size_t)nw * kRounds, 0.f);
+    std::vector<bool> same(nw, true);
+    for (int round = 0; round <= kRounds && e == xmpErrorSuccess; round++) {
+      for (uint32_t k = 0; k < nw; k++) {
+        args.window_bits = wlo + k;
         (void)hipEventRecord(ev0, handle->stream);
         (void)launch_powm_variant(vi, args, handle->stream);
         (void)hipEventRecord(ev1, handle->stream);
         (void)hipEventSynchronize(ev1);
         float ms = 0;
         (void)hipEventElapsedTime(&ms, ev0, ev1);
-        if (rep > 0) times[rep - 1] = ms;
+        if (round > 0) times[(size_t)k * kRounds + round - 1] = ms;
+        if (round == kRounds) {
+          // self-check: each width's results against the gold standard
+          words = n32;
+          if ((e = xmpIntegersExport(handle, test.data(), &words, -1, 4, 0, 0, O, count)) != xmpErrorSuccess) break;
+          same[k] = test == gold;
+        }
       }
-      handle->df(table);
-      std::sort(times, times + 3);
-      const float best_ms = times[1];
-      // self-check: the timed layout's results against the gold standard
-      words = n32;
-      if ((e = xmpIntegersExport(handle, test.data(), &words, -1, 4, 0, 0, O, count)) != xmpErrorSuccess) break;
-      if (test != gold) {
+    }
+    handle->df(table);
+    for (uint32_t k = 0; k < nw && e == xmpErrorSuccess; k++) {
+      if (!same[k]) {
         mismatched++;
         continue;
       }
+      float *tk = times.data() + (size_t)k * kRounds;
+      std::sort(tk, tk + kRounds);
+      const float med_ms = 0.5f * (tk[kRounds / 2 - 1] + tk[kRounds / 2]);
       nmeasured++;
-      double rate = count / (best_ms * 1e-3);
+      const uint32_t w = wlo + k;
+      double rate = count / (med_ms * 1e-3);
       char buf[512];
       std::snprintf(buf, sizeof buf,
                     "{\"precision\": %u, \"exp_bits\": %u, \"count\": %u, \"lanes\": %d, \"digits\": %d, "
                     "\"radix\": %d, \"window\": %u, \"ms\": %.4f, \"powm_per_s\": %.1f}",
-                    precision, exp_precision, count, v.lanes, v.digits, v.radix, w, best_ms, rate);
+                    precision, exp_precision, count, v.lanes, v.digits, v.radix, w, med_ms, rate);
       if (!out.empty()) out += ",\n";
       out += buf;
       if (rate > best_rate) {
