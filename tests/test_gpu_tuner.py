@@ -56,14 +56,14 @@ def _entry(bits, count, lanes, digits, ms, window=6):
             "window": window, "ms": ms, "powm_per_s": count / (ms * 1e-3)}
 
 
-@pytest.mark.parametrize("count,want", [(1000, (16, 3, 6)), (4096, (16, 3, 6)), (8192, (4, 10, 6)),
-                                        (16384, (4, 10, 5)), (100000, (4, 10, 5))])
+@pytest.mark.parametrize("count,want", [(1000, (16, 3, 6)), (4096, (16, 3, 6)), (4097, (4, 10, 5)),
+                                        (8192, (4, 10, 5)), (16384, (4, 10, 5)), (100000, (4, 10, 5))])
 def test_choice_between_measured_batch_sizes(handle, count, want):
-    """Between two measured batch sizes each layout's time is interpolated on a
-    log-log scale: 16x3 is faster at 4,096 (2.4 against 2.8 ms) and 4x10 at
-    16,384 (2.84 against 6.0 ms), so 8,192, halfway in log scale, goes to 4x10
-    (about 2.82 against 3.8 ms), with the window measured at 4,096 (ties go to
-    the smaller size); outside the measured sizes the nearest size decides."""
+    """A batch between two measured sizes is priced at the larger one: 16x3 is
+    faster at 4,096 (2.4 against 2.8 ms) and 4x10 at 16,384 (2.84 against
+    6.0 ms), so every batch of 4,097 to 16,384 goes to 4x10 with the window
+    measured at 16,384; below the smallest measured size the smallest decides,
+    above the largest the largest."""
     rows = [_entry(1024, 4096, 16, 3, 2.4, 6), _entry(1024, 4096, 4, 10, 2.8, 6),
             _entry(1024, 16384, 16, 3, 6.0, 6), _entry(1024, 16384, 4, 10, 2.84, 5)]
     shipped = os.path.join(os.path.dirname(xmp_amd.LIB_PATH), "tune_gfx950.json")

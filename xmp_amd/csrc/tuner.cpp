@@ -141,12 +141,15 @@ xmpError_t choose_powm(xmpHandle_t h, uint32_t precision, uint32_t exp_bits, uin
     if (alg == xmpAlgorithmDistributedMP) return v.lanes > 1;
     return true;
   };
-  // Measured entries for this precision, at the exponent size measured nearest
-  // the request's.  Each layout's time for `count` instances is read from its
-  // times at the measured batch sizes either side, interpolated on a log-log
-  // scale (a batch of 8,192 between sizes 4,096 and 16,384 is priced from
-  // both, not from whichever is nearer); outside the measured sizes the
-  // nearest size stands in.  The fastest allowed layout wins.
+  // Measured entries for this precision, at the exponent size measured
+  // nearest the request's, taken at the smallest measured batch size that is
+  // at least `count` (the largest one for batches beyond every measured
+  // size); the fastest allowed layout there wins.  A layout's time grows in
+  // steps as its waves outgrow the resident wave slots, so a batch between
+  // two measured sizes can cost up to the larger size's time: pricing it at
+  // that bound picks the layout whose worst case is best.  (Interpolating
+  // log-log between the two sizes sent 12,936 8192-bit instances to 16x19,
+  // 772 ms, where 8x37 takes about 742.)
   {
     uint32_t eb = 0;
     double ebd = 1e30;
@@ -158,35 +161,19 @@ xmpError_t choose_powm(xmpHandle_t h, uint32_t precision, uint32_t exp_bits, uin
         eb = e.exp_bits;
       }
     }
-    uint32_t lo = 0, hi = 0;   // largest measured size <= count, smallest >= count
+    uint32_t lo = 0, hi = 0;   // largest measured size below count, smallest at or above it
     for (const TunedEntry &e : g_table) {
       if (e.precision != precision || e.exp_bits != eb || !eb) continue;
-      if (e.count <= count) lo = std::max(lo, e.count);
+      if (e.count < count) lo = std::max(lo, e.count);
       if (e.count >= count) hi = hi ? std::min(hi, e.count) : e.count;
     }
-    if (!lo) lo = hi;
-    if (!hi) hi = lo;
+    const uint32_t size = hi ? hi : lo;
     const TunedEntry *pick = nullptr;
-    double pick_t = 1e300;
     for (const TunedEntry &a : g_table) {
-      if (!lo || a.precision != precision || a.exp_bits != eb || a.count != lo || a.rate <= 0) continue;
+      if (!size || a.precision != precision || a.exp_bits != eb || a.count != size || a.rate <= 0) continue;
       int vi = powm_find_variant(a.lanes, a.digits, a.radix);
       if (vi < 0 || !allowed(powm_variant(vi))) continue;
-      const TunedEntry *b = hi == lo ? &a : nullptr;
-      for (const TunedEntry &c : g_table)
-        if (!b && c.precision == precision && c.exp_bits == eb && c.count == hi && c.rate > 0 && c.lanes == a.lanes &&
-            c.digits == a.digits && c.radix == a.radix)
-          b = &c;
-      if (!b) continue;
-      const double ta = std::log((double)a.count / a.rate), tb = std::log((double)b->count / b->rate);
-      const double f = hi == lo ? 0.0
-                                : (std::log((double)count) - std::log((double)lo)) /
-                                      (std::log((double)hi) - std::log((double)lo));
-      const double t = ta + (tb - ta) * f;
-      if (t < pick_t) {
-        pick_t = t;
-        pick = f <= 0.5 ? &a : b;   // the window measured at the nearer size
-      }
+      if (!pick || a.rate > pick->rate) pick = &a;
     }
     if (pick) {
       out->variant = powm_find_variant(pick->lanes, pick->digits, pick->radix);

@@ -7,9 +7,8 @@ Here, for each precision class and batch size, xmpTunerRun times every lane
 layout (lanes x digits x radix) and the window widths around the modelled best
 on random per-instance moduli; the table is written as JSON next to the
 library (xmp_amd/tune_gfx950.json), where the runtime loads it by default
-($XMP_TUNE_FILE overrides).  The choice for a call is the layout with the
-shortest time for its batch size, each layout's time interpolated (log-log)
-between the measured batch sizes either side.
+($XMP_TUNE_FILE overrides).  The choice for a call is the fastest layout at
+the smallest measured batch size that holds its batch.
 
 Usage (on the GPU):  python -m xmp_amd.tune [--out PATH] [--quick]
 """
