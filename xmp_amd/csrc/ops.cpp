@@ -261,13 +261,16 @@ xmpError_t powm_run(xmpHandle_t handle, xmpIntegers_t out, xmpIntegers_t a, xmpI
   const uint32_t mstride = round_up(nmod, kStrideAlign);
   const size_t r2_words = (size_t)mod->nlimbs * mstride;
   const size_t div_words = mod->nlimbs > kMaxRegDivisorLimbs ? divmod_scratch_words(mod->nlimbs, nmod) : 0;
+  // the memory tier's R^2 squarings need a block per modulus of their own
+  const size_t r2blk_words = wide ? powm_wide_r2_scratch_words(wide, nmod) : 0;
   const size_t obuf_words = ws ? round_up((uint32_t)(obytes / sizeof(uint32_t)), 64) : 0;
   const size_t limit = ws ? ws->bytes : (p->scratch_size_limit ? p->scratch_size_limit : handle->memorySize / 4 * 3);
-  const size_t fixed = (r2_words + div_words + obuf_words) * sizeof(uint32_t);
+  const size_t fixed = (r2_words + div_words + r2blk_words + obuf_words) * sizeof(uint32_t);
   uint32_t wbits = c.window_bits;
   uint32_t chunk = count;
   while (fixed + table_words(chunk, wbits) * sizeof(uint32_t) > limit) {
-    if (!ws && chunk > 65536) {
+    // (a workgroup-layout instance can need megabytes: its chunks may shrink further)
+    if (!ws && chunk > (wide ? kStrideAlign : 65536u)) {
       chunk = round_up(chunk / 2, kStrideAlign);
     } else if (ws && wbits > 3) {
       wbits--;
@@ -284,7 +287,7 @@ xmpError_t powm_run(xmpHandle_t handle, xmpIntegers_t out, xmpIntegers_t a, xmpI
   uint32_t *obuf = out->limbs;
   if (ws) {
     base = (uint32_t *)ws->base;
-    if (inplace) obuf = base + r2_words + div_words;
+    if (inplace) obuf = base + r2_words + div_words + r2blk_words;
   } else {
     e = set_necessary_scratch(handle, need);
     if (e != xmpErrorSuccess) return e;
@@ -301,9 +304,10 @@ xmpError_t powm_run(xmpHandle_t handle, xmpIntegers_t out, xmpIntegers_t a, xmpI
     return xmpErrorCuda;
   uint32_t *r2 = base;
   uint32_t *divws = r2 + r2_words;
-  uint32_t *table = divws + div_words + obuf_words;
+  uint32_t *r2blk = divws + div_words;
+  uint32_t *table = r2blk + r2blk_words + obuf_words;
   if ((wide ? launch_wide_r2(wide, r2, mstride, mod->limbs, mod->nlimbs, mod->stride, nmod,
-                             div_words ? divws : nullptr, handle->stream)
+                             div_words ? divws : nullptr, r2blk_words ? r2blk : nullptr, handle->stream)
             : launch_montgomery_r2(c.variant, r2, mstride, mod->limbs, mod->nlimbs, mod->stride, nmod,
                                    div_words ? divws : nullptr, handle->stream)) != hipSuccess)
     return xmpErrorCuda;

@@ -78,16 +78,22 @@ hipError_t launch_r2_canonical(uint32_t *r2, uint32_t r2_stride, const uint32_t 
                                const uint32_t *m, uint32_t m_len, uint32_t m_stride, uint32_t count,
                                hipStream_t stream);
 
-// Moduli wider than every lane layout: one 256-thread workgroup per instance
-// with C radix-2^24 digits per thread (C = 8, 16, 32; R = 2^(6144 C)).
-// digits per thread for a precision, or 0 above the widest (196,606 bits)
+// Moduli wider than every lane layout: one workgroup per instance.  A tier
+// is its digit count in units of 256: 8, 16, 32, 64 keep radix-2^24 digits
+// in registers (up to 393,214 bits), 4C (C >= 20) keeps C radix-2^20 digits
+// per thread of 1024 in global memory (up to 83,886,078 bits).
+// the tier for a precision, or 0 above the widest
 uint32_t powm_wide_digits_per_thread(uint32_t precision);
-uint32_t powm_wide_rbits(uint32_t c);
-size_t powm_wide_table_words(uint32_t c, uint32_t count, uint32_t window_bits);
-hipError_t launch_powm_wide(uint32_t c, const PowmArgs &args, hipStream_t stream);
-// R^2 mod m for the wide layout (needs `ws` of divmod_scratch_words(m_len, count) words)
-hipError_t launch_wide_r2(uint32_t c, uint32_t *r2, uint32_t r2_stride, const uint32_t *m, uint32_t m_len,
-                          uint32_t m_stride, uint32_t count, uint32_t *ws, hipStream_t stream);
+uint32_t powm_wide_rbits(uint32_t tier);
+// the table workspace (for the memory tier, each instance's whole block)
+size_t powm_wide_table_words(uint32_t tier, uint32_t count, uint32_t window_bits);
+// the memory tier's blocks for R^2 set-up over `count` moduli (0 for register tiers)
+size_t powm_wide_r2_scratch_words(uint32_t tier, uint32_t count);
+hipError_t launch_powm_wide(uint32_t tier, const PowmArgs &args, hipStream_t stream);
+// R^2 mod m for the wide layout (needs `ws` of divmod_scratch_words(m_len, count)
+// words, and for the memory tier `tier_ws` of powm_wide_r2_scratch_words words)
+hipError_t launch_wide_r2(uint32_t tier, uint32_t *r2, uint32_t r2_stride, const uint32_t *m, uint32_t m_len,
+                          uint32_t m_stride, uint32_t count, uint32_t *ws, uint32_t *tier_ws, hipStream_t stream);
 // variants usable for a modulus precision (in bits): those with R >= 2^(precision+2)
 bool variant_fits(const PowmVariant &v, uint32_t precision);
 // whether the persistent mixed-size kernel runs this variant
