@@ -74,22 +74,29 @@ def test_powm_wider_than_lane_layouts(handle, bits, exp_bits, n):
     assert_rows_equal(got, gmp(O.GO_POWM, n, bits // 32, b, e, m), "powm %d-bit, %d-bit exponent" % (bits, exp_bits))
 
 
-@pytest.mark.parametrize("bits,exp_bits,n", [(393216, 32, 2), (409600, 64, 1), (524288, 32, 1)])
+@pytest.mark.parametrize("bits,exp_bits,n", [(393216, 32, 4), (409600, 64, 1), (524288, 32, 1)])
 def test_powm_memory_tier(handle, bits, exp_bits, n):
     """Above 393,214 bits the digits live in global memory (1024 threads, C
     radix-2^20 digits each: C = 20, 21 and 26 here); against GMP, with the
-    first instance's base at m - 1 and an exponent of all ones."""
+    first instance's base at m - 1 and an exponent of all ones, and at
+    393,216 bits also e = 0 and an even modulus (defined as 0)."""
     limbs = bits // 32
     m = inputs.odd_moduli(3300 + bits, n, limbs)
     b = inputs.below(3301 + bits, m)
-    e = inputs.uniform(3302 + bits, n, exp_bits // 32 or 1)
+    e = inputs.uniform(3302 + bits, n, exp_bits // 32)
     b[0] = m[0]
     b[0, 0] -= 1
     e[0] = 0xFFFFFFFF
+    if n > 2:
+        e[2] = 0
+        m[3, 0] &= 0xFFFFFFFE
     M, B, E = handle.from_array(m), handle.from_array(b), handle.from_array(e)
     Out = handle.integers(bits, n)
     handle.powm(Out, B, E, M)
-    assert_rows_equal(Out.get(), gmp(O.GO_POWM, n, limbs, b, e, m), "powm %d-bit in global memory" % bits)
+    want = gmp(O.GO_POWM, n, limbs, b, e, m)
+    if n > 2:
+        want[3] = 0
+    assert_rows_equal(Out.get(), want, "powm %d-bit in global memory" % bits)
 
 
 def test_powm_beyond_the_widest_tier(handle):
