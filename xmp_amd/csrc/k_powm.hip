@@ -98,9 +98,9 @@ struct Mont {
   // crosses lanes.
   __device__ __forceinline__ void step(uint64_t (&t)[K], const uint32_t (&a)[K], uint32_t bi) const {
     uint64_t f = t[0] + (uint64_t)a[0] * bi;
-    const uint32_t q = g.bcast0(mul_lo_mad((uint32_t)f, np0)) & mask;
+    const uint32_t q = g.bcast0_and(mul_lo_mad((uint32_t)f, np0), mask);
     f += (uint64_t)q * m[0];
-    const uint32_t up = g.from_next_z((uint32_t)f) & mask;
+    const uint32_t up = g.next_digit((uint32_t)f, mask);
 #pragma unroll
     for (int j = 1; j < K; j++) t[j - 1] = t[j] + (uint64_t)a[j] * bi + (uint64_t)q * m[j];
     t[K - 1] = up;
@@ -114,9 +114,9 @@ struct Mont {
   __device__ __forceinline__ void step2(uint64_t (&t)[K], const uint32_t (&a)[K], uint32_t bi,
                                         const uint32_t (&c)[K], uint32_t di) const {
     uint64_t f = t[0] + (uint64_t)a[0] * bi + (uint64_t)c[0] * di;
-    const uint32_t q = g.bcast0(mul_lo_mad((uint32_t)f, np0)) & mask;
+    const uint32_t q = g.bcast0_and(mul_lo_mad((uint32_t)f, np0), mask);
     f += (uint64_t)q * m[0];
-    const uint32_t up = g.from_next_z((uint32_t)f) & mask;
+    const uint32_t up = g.next_digit((uint32_t)f, mask);
 #pragma unroll
     for (int j = 1; j < K; j++)
       t[j - 1] = t[j] + (uint64_t)a[j] * bi + (uint64_t)c[j] * di + (uint64_t)q * m[j];
@@ -304,9 +304,9 @@ struct Mont {
     uint64_t f = t[0];
     if constexpr (KI == 0) f += (uint64_t)a[0] * bi;
     else if constexpr (beats(KI, 0)) f += twice(a, a2, 0, bi, b2);
-    const uint32_t q = g.bcast0(mul_lo_mad((uint32_t)f, np0)) & mask;
+    const uint32_t q = g.bcast0_and(mul_lo_mad((uint32_t)f, np0), mask);
     f += (uint64_t)q * m[0];
-    const uint32_t up = g.from_next_z((uint32_t)f) & mask;
+    const uint32_t up = g.next_digit((uint32_t)f, mask);
     gsqr_cols<KI>(t, a, a2, q, bi, b2, std::make_integer_sequence<int, K - 1>());
     t[K - 1] = up;
     t[0] += f >> D;
