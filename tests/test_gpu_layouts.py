@@ -82,6 +82,45 @@ def test_every_layout(handle, layout):
         _restore()
 
 
+# Layouts with a quotient-free kernel (the reduction by m * (-m^-1 mod 2^D),
+# taken when the active lanes hold D bits more than R >= 4m needs): the
+# largest size that takes it and the largest that does not.
+QUOTIENT_FREE = {
+    (1, 20, 28): (512, 544), (2, 19, 28): (1024, 1056), (2, 28, 28): (1536, 768), (4, 19, 28): (2080, 2112),
+    (4, 28, 28): (3104, 2336), (8, 29, 27): (6208, 6240), (32, 20, 26): (16608, 16096),
+    (64, 20, 26): (33248, 32736), (8, 37, 28): (8256, 7232), (4, 10, 28): (1088, 832), (8, 5, 28): (1088, 960),
+    (8, 10, 28): (2208, 1952), (16, 5, 28): (2208, 2080), (16, 3, 28): (1312, 1248),
+}
+
+
+def _spare_digit(layout, bits):
+    """Whether the lanes holding a bits-wide modulus have D bits to spare."""
+    lanes, digits, radix = layout
+    nd = -(-(bits + 2) // radix)
+    active = min(lanes, -(-nd // digits))
+    return digits * radix * active >= bits + 2 + radix
+
+
+@pytest.mark.parametrize("layout", sorted(QUOTIENT_FREE))
+def test_quotient_free_boundary(handle, layout):
+    """Each such layout on both sides of the boundary, against GMP."""
+    lanes = layout[0]
+    n = max(64 // lanes, 1) * 2 + 1
+    on, off = QUOTIENT_FREE[layout]
+    assert _spare_digit(layout, on) and not _spare_digit(layout, off)
+    try:
+        for bits in (on, off):
+            m, b, e = _case(500 + bits + lanes, n, bits)
+            want = gmp(O.GO_POWM, n, bits // 32, b, e, m)
+            _load([_entry(layout, bits, n)])
+            M, B, E = handle.from_array(m), handle.from_array(b), handle.from_array(e)
+            out = handle.integers(bits, n)
+            handle.powm(out, B, E, M)
+            assert_rows_equal(out.get(), want, "%dx%d radix %d, %d bits" % (layout + (bits,)))
+    finally:
+        _restore()
+
+
 @pytest.mark.parametrize("layouts", [((4, 37, 28), (8, 38, 27)), ((4, 38, 27), (16, 19, 27))])
 def test_batch_on_wide_layouts(handle, layouts):
     """The persistent batch with a 4096-bit and an 8192-bit class: the 4x37

@@ -70,6 +70,16 @@ __device__ __forceinline__ uint32_t mul_lo_mad(uint32_t x, uint32_t y) {
   return (uint32_t)r;
 }
 
+// np0 = -m0^-1 mod 2^D for odd m0 (Newton's iteration doubles the correct
+// low bits, from 3 for inv = m0: reference regmp/np0.h:24-33)
+template <int D>
+__device__ __forceinline__ uint32_t neg_inverse(uint32_t m0) {
+  uint32_t inv = m0;
+#pragma unroll
+  for (int it = 0; it < 4; it++) inv = inv * (2u - m0 * inv);
+  return (0u - inv) & ((1u << D) - 1);
+}
+
 template <int L, int K, int D>
 struct Mont {
   static constexpr int ND = L * K;
@@ -96,15 +106,29 @@ struct Mont {
   // f_hi stays at this lane's new position 0.  For the group's first lane
   // f_lo = 0, so the code is the same on every lane and only a 28-bit value
   // crosses lanes.
+  template <bool QF = false>
   __device__ __forceinline__ void step(uint64_t (&t)[K], const uint32_t (&a)[K], uint32_t bi) const {
     uint64_t f = t[0] + (uint64_t)a[0] * bi;
-    const uint32_t q = g.bcast0_and(mul_lo_mad((uint32_t)f, np0), mask);
+    const uint32_t q = quotient<QF>(f);
     f += (uint64_t)q * m[0];
     const uint32_t up = g.next_digit((uint32_t)f, mask);
 #pragma unroll
     for (int j = 1; j < K; j++) t[j - 1] = t[j] + (uint64_t)a[j] * bi + (uint64_t)q * m[j];
     t[K - 1] = up;
     t[0] += f >> D;
+  }
+
+  // The step's quotient digit, the group's first lane's q = f * np0 mod 2^D,
+  // on every lane of the group.  Quotient-free (QF): m holds a multiple
+  // m~ = m * np0 of the modulus, which is -1 mod 2^D, so q = f mod 2^D.
+  template <bool QF>
+  __device__ __forceinline__ uint32_t quotient(uint64_t f) const {
+    if constexpr (QF) {
+      uint32_t x = (uint32_t)f;
+      asm("" : "+v"(x));
+      return g.bcast0_and(x, mask);
+    }
+    return g.bcast0_and(mul_lo_mad((uint32_t)f, np0), mask);
   }
 
   // One CIOS step of a sum of two products: t = (t + a*b_i + c*d_i + q*m) / 2^D.
@@ -114,7 +138,7 @@ struct Mont {
   __device__ __forceinline__ void step2(uint64_t (&t)[K], const uint32_t (&a)[K], uint32_t bi,
                                         const uint32_t (&c)[K], uint32_t di) const {
     uint64_t f = t[0] + (uint64_t)a[0] * bi + (uint64_t)c[0] * di;
-    const uint32_t q = g.bcast0_and(mul_lo_mad((uint32_t)f, np0), mask);
+    const uint32_t q = quotient<false>(f);
     f += (uint64_t)q * m[0];
     const uint32_t up = g.next_digit((uint32_t)f, mask);
 #pragma unroll
@@ -128,13 +152,15 @@ struct Mont {
   // LDS, of which the first nl*K are the multiplier's (R = 2^(D*K*nl)).  With
   // every lane's digits the step count is a constant, as the compiler best
   // schedules it; otherwise a run-time count.
+  template <bool QF = false>
   __device__ __forceinline__ void mul(uint32_t (&r)[K], const uint32_t (&a)[K], const uint32_t *b) const {
     if (L == 1 || nl == (uint32_t)L)
-      mul_steps(r, a, b, ND);
+      mul_steps<QF>(r, a, b, ND);
     else
-      mul_steps(r, a, b, (int)nl * K);
+      mul_steps<QF>(r, a, b, (int)nl * K);
   }
 
+  template <bool QF>
   __device__ __forceinline__ void mul_steps(uint32_t (&r)[K], const uint32_t (&a)[K], const uint32_t *b,
                                             const int ns) const {
     uint64_t t[K];
@@ -143,15 +169,15 @@ struct Mont {
     const int ns4 = ns / 4 * 4;
     for (int i = 0; i < ns4; i += 4) {
       uint4 bb = *reinterpret_cast<const uint4 *>(b + i);
-      step(t, a, bb.x);
-      step(t, a, bb.y);
-      step(t, a, bb.z);
-      step(t, a, bb.w);
+      step<QF>(t, a, bb.x);
+      step<QF>(t, a, bb.y);
+      step<QF>(t, a, bb.z);
+      step<QF>(t, a, bb.w);
       if constexpr (kCarryPasses)
         if ((i + 4) % 64 == 0) carry_pass(t);
     }
 #pragma unroll 3
-    for (int i = ns4; i < ns; i++) step(t, a, b[i]);
+    for (int i = ns4; i < ns; i++) step<QF>(t, a, b[i]);
     normalize(r, t);
   }
 
@@ -187,28 +213,29 @@ struct Mont {
   // those of the plain product.  ND(ND+1)/2 + ND^2 multiply-adds instead of
   // 2 ND^2.  Bound per accumulator: (ND/2+1) doubled products < 2^(2D+1) plus
   // ND products < 2^(2D), the same 2^(2D)(2ND+2) as the plain product.
+  template <bool QF = false>
   __device__ __forceinline__ void sqr(uint32_t (&r)[K], const uint32_t (&a)[K]) const {
     static_assert(L == 1, "symmetric squaring needs the whole instance in one lane");
     uint64_t t[K];
 #pragma unroll
     for (int k = 0; k < K; k++) t[k] = 0;
     // one instantiation per step, so every register index is a constant
-    sqr_steps(t, a, std::make_integer_sequence<int, K>());
+    sqr_steps<QF>(t, a, std::make_integer_sequence<int, K>());
     normalize(r, t);
   }
 
-  template <int... I>
+  template <bool QF, int... I>
   __device__ __forceinline__ void sqr_steps(uint64_t (&t)[K], const uint32_t (&a)[K],
                                             std::integer_sequence<int, I...>) const {
-    (sqr_step<I>(t, a), ...);
+    (sqr_step<QF, I>(t, a), ...);
   }
 
-  template <int I>
+  template <bool QF, int I>
   __device__ __forceinline__ void sqr_step(uint64_t (&t)[K], const uint32_t (&a)[K]) const {
     const uint32_t ai = a[I], ai2 = a[I] << 1;
     uint64_t f = t[0];
     if constexpr (I == 0) f += (uint64_t)ai * ai;
-    const uint32_t q = mul_lo_mad((uint32_t)f, np0) & MASK;
+    const uint32_t q = (QF ? (uint32_t)f : mul_lo_mad((uint32_t)f, np0)) & MASK;
     f += (uint64_t)q * m[0];
 #pragma unroll
     for (int j = 1; j < K; j++) {
@@ -238,6 +265,7 @@ struct Mont {
   // the plain product when it is retired: quotients, carries and the result
   // are those of mul(r, a, b), bit for bit, with the same accumulator bound.
   // Per step K + floor(K/2) + 1 multiply-adds per lane instead of 2K.
+  template <bool QF = false>
   __device__ __forceinline__ void sqr_group(uint32_t (&r)[K], const uint32_t (&a)[K], const uint32_t *b) const {
     uint64_t t[K];
 #pragma unroll
@@ -265,7 +293,7 @@ struct Mont {
         ac[k] = a[k];
         if constexpr (kCopyBarrier) asm("" : "+v"(ac[k]));
       }
-      gsqr_steps(t, ac, a2, b + li * K, std::make_integer_sequence<int, K>());
+      gsqr_steps<QF>(t, ac, a2, b + li * K, std::make_integer_sequence<int, K>());
       if constexpr (kCarryPasses)
         if ((li + 1) % kPassLanes == 0 && li + 1 < nlr) carry_pass(t);
     }
@@ -284,10 +312,10 @@ struct Mont {
     return 2 * d < K || (2 * d == K && x < K / 2);
   }
 
-  template <int... I>
+  template <bool QF, int... I>
   __device__ __forceinline__ void gsqr_steps(uint64_t (&t)[K], const uint32_t (&a)[K], const uint32_t (&a2)[K],
                                              const uint32_t *bl, std::integer_sequence<int, I...>) const {
-    (gsqr_step<I>(t, a, a2, bl), ...);
+    (gsqr_step<QF, I>(t, a, a2, bl), ...);
   }
 
   // 2 a_j b_i, from the doubled own digit or the doubled multiplier digit
@@ -296,7 +324,7 @@ struct Mont {
     return kOwnDoubled ? (uint64_t)a2[j] * bi : (uint64_t)a[j] * b2;
   }
 
-  template <int KI>
+  template <bool QF, int KI>
   __device__ __forceinline__ void gsqr_step(uint64_t (&t)[K], const uint32_t (&a)[K], const uint32_t (&a2)[K],
                                             const uint32_t *bl) const {
     const uint32_t bi = bl[KI];
@@ -304,7 +332,7 @@ struct Mont {
     uint64_t f = t[0];
     if constexpr (KI == 0) f += (uint64_t)a[0] * bi;
     else if constexpr (beats(KI, 0)) f += twice(a, a2, 0, bi, b2);
-    const uint32_t q = g.bcast0_and(mul_lo_mad((uint32_t)f, np0), mask);
+    const uint32_t q = quotient<QF>(f);
     f += (uint64_t)q * m[0];
     const uint32_t up = g.next_digit((uint32_t)f, mask);
     gsqr_cols<KI>(t, a, a2, q, bi, b2, std::make_integer_sequence<int, K - 1>());
@@ -359,8 +387,8 @@ __device__ __forceinline__ uint32_t exp_bits(const Operand &e, uint32_t inst, ui
 // One tile: the 64/L instances [tile*64/L, tile*64/L + 64/L) of a launch,
 // processed by one wave.  `bbuf` is this wave's LDS staging area
 // (G * NDP words, 16-byte aligned).
-template <int L, int K, int D>
-__device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, uint32_t *bbuf) {
+template <int L, int K, int D, bool QF>
+__device__ __forceinline__ void powm_tile_mode(const PowmArgs &args, uint32_t tile, uint32_t *bbuf) {
   using M = Mont<L, K, D>;
   constexpr int ND = M::ND;
   constexpr int NDP = (ND + 3) / 4 * 4;
@@ -386,13 +414,13 @@ __device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, u
     mont.m[k] = limb_digit_guarded<D>(args.mod.data, args.mod.stride, mi, args.mod.len, d);
     b[l * K + k] = limb_digit_guarded<D>(args.r2, args.r2_stride, mi, args.mod.len, d);
   }
-  {
-    // np0 = -m^-1 mod 2^D (Newton, as reference regmp/np0.h:24-33)
-    uint32_t m0 = ld_limb(args.mod, mi, 0);
-    uint32_t inv = m0;
+  mont.np0 = neg_inverse<D>(ld_limb(args.mod, mi, 0));
+  if constexpr (QF) {
+    // m~ = m * np0 < 2^(32*len + D), which the active lanes hold
+    uint64_t t[K];
 #pragma unroll
-    for (int it = 0; it < 4; it++) inv = inv * (2u - m0 * inv);
-    mont.np0 = (0u - inv) & M::MASK;
+    for (int k = 0; k < K; k++) t[k] = (uint64_t)mont.m[k] * mont.np0;
+    normalize_group<L, K, D>(mont.m, t, mont.g);
   }
 
   const uint32_t wbits = args.window_bits;
@@ -404,14 +432,14 @@ __device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, u
 #pragma unroll
   for (int k = 0; k < K; k++) a[k] = (l == 0 && k == 0) ? 1u : 0u;
   wave_sync();
-  mont.mul(a, a, b);
+  mont.template mul<QF>(a, a, b);
 #pragma unroll
   for (int k = 0; k < K; k++) *tab_at(0, k) = a[k];
   // slot 1 = base * R mod m
 #pragma unroll
   for (int k = 0; k < K; k++)
     a[k] = limb_digit_guarded<D>(args.base.data, args.base.stride, bi, args.base.len, (uint32_t)(l * K + k));
-  mont.mul(a, a, b);
+  mont.template mul<QF>(a, a, b);
   wave_sync();
   // slots 2..T-1: slot[e] = slot[e-1] * slot[1]
 #pragma unroll
@@ -421,7 +449,7 @@ __device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, u
   }
   wave_sync();
   for (uint32_t e = 2; e < T; e++) {
-    mont.mul(a, a, b);
+    mont.template mul<QF>(a, a, b);
 #pragma unroll
     for (int k = 0; k < K; k++) *tab_at(e, k) = a[k];
   }
@@ -438,23 +466,34 @@ __device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, u
     idx = exp_bits(args.exp, ei, pos, wbits);
     for (uint32_t s = 0; s < wbits; s++) {
       if constexpr (L == 1) {
-        mont.sqr(a, a);
+        mont.template sqr<QF>(a, a);
       } else {
 #pragma unroll
         for (int k = 0; k < K; k++) b[l * K + k] = a[k];
         wave_sync();
-        mont.sqr_group(a, a, b);
+        mont.template sqr_group<QF>(a, a, b);
         wave_sync();
       }
     }
 #pragma unroll
     for (int k = 0; k < K; k++) b[l * K + k] = *tab_at(idx, k);
     wave_sync();
-    mont.mul(a, a, b);
+    mont.template mul<QF>(a, a, b);
     wave_sync();
   }
 #undef tab_at
-  // leave Montgomery form: a * 1 / R, which is <= m
+  // leave Montgomery form: a * 1 / R, which is <= m.  Quotient-free, a is
+  // below 2 m~ and congruent to the result mod m, and this one product
+  // reduces by m itself: (a + Q m) / R < 2 m~ / R + m <= m + 1/2.
+  if constexpr (QF) {
+    // the load addresses are recomputed here (an opaque copy of the index),
+    // not kept live from the prologue through the exponentiation
+    uint32_t mr = mi, lr = (uint32_t)(l * K), sr = args.mod.stride;
+    asm volatile("" : "+v"(mr), "+v"(lr), "+s"(sr));
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      mont.m[k] = limb_digit_guarded<D>(args.mod.data, sr, mr, args.mod.len, lr + (uint32_t)k);
+  }
 #pragma unroll
   for (int k = 0; k < K; k++) b[l * K + k] = (l == 0 && k == 0) ? 1u : 0u;
   wave_sync();
@@ -474,15 +513,26 @@ __device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, u
   if (active) store_limbs<L, D>(b, ND, args.out.data, args.out.stride, out_index(args.out, gi), args.out.len, l);
 }
 
+// the persistent batch's tiles take the quotient of every step
+template <int L, int K, int D>
+__device__ __forceinline__ void powm_tile(const PowmArgs &args, uint32_t tile, uint32_t *bbuf) {
+  powm_tile_mode<L, K, D, false>(args, tile, bbuf);
+}
+
 template <int L, int K, int D>
 constexpr int lds_words() {
   return (64 / L) * ((L * K + 3) / 4 * 4);
 }
 
-template <int L, int K, int D>
+// One wave per tile.  The quotient-free form (QF) is a kernel of its own, so
+// that each form gets its own register allocation; the launcher takes it
+// when the active lanes have D bits to spare (powm_quotient_free): one
+// multiply-add fewer per step, the reduction running by m~ = m * np0, one
+// digit longer than m, at the same R.
+template <int L, int K, int D, bool QF = false>
 __global__ void __launch_bounds__(64, 2) powm_kernel(PowmArgs args) {
   __shared__ __attribute__((aligned(16))) uint32_t bbuf[lds_words<L, K, D>()];
-  powm_tile<L, K, D>(args, blockIdx.x, bbuf);
+  powm_tile_mode<L, K, D, QF>(args, blockIdx.x, bbuf);
 }
 
 #ifdef XMP_POWM_LATENCY_UNIT
@@ -491,19 +541,27 @@ __global__ void __launch_bounds__(64, 2) powm_kernel(PowmArgs args) {
 // The few-digit latency layouts, compiled in a unit of their own
 // (k_powm_latency.hip) with the default instruction scheduler, which issues
 // their short steps 1-3 % faster than the ILP-oriented one used for this file.
+template <int L, int K>
+void launch_latency_kernel(dim3 grid, const PowmArgs &args, hipStream_t stream) {
+  if (powm_quotient_free(L, K, 28, args.mod.len))
+    hipLaunchKernelGGL((powm_kernel<L, K, 28, true>), grid, dim3(64), 0, stream, args);
+  else
+    hipLaunchKernelGGL((powm_kernel<L, K, 28>), grid, dim3(64), 0, stream, args);
+}
+
 hipError_t launch_powm_latency(int lanes, int digits, const PowmArgs &args, hipStream_t stream) {
   if (args.count == 0) return hipErrorInvalidValue;
-  const dim3 grid(div_round_up(args.count, 64u / (uint32_t)lanes)), block(64);
+  const dim3 grid(div_round_up(args.count, 64u / (uint32_t)lanes));
   if (lanes == 4 && digits == 10)
-    hipLaunchKernelGGL((powm_kernel<4, 10, 28>), grid, block, 0, stream, args);
+    launch_latency_kernel<4, 10>(grid, args, stream);
   else if (lanes == 8 && digits == 5)
-    hipLaunchKernelGGL((powm_kernel<8, 5, 28>), grid, block, 0, stream, args);
+    launch_latency_kernel<8, 5>(grid, args, stream);
   else if (lanes == 8 && digits == 10)
-    hipLaunchKernelGGL((powm_kernel<8, 10, 28>), grid, block, 0, stream, args);
+    launch_latency_kernel<8, 10>(grid, args, stream);
   else if (lanes == 16 && digits == 5)
-    hipLaunchKernelGGL((powm_kernel<16, 5, 28>), grid, block, 0, stream, args);
+    launch_latency_kernel<16, 5>(grid, args, stream);
   else if (lanes == 16 && digits == 3)
-    hipLaunchKernelGGL((powm_kernel<16, 3, 28>), grid, block, 0, stream, args);
+    launch_latency_kernel<16, 3>(grid, args, stream);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -514,34 +572,40 @@ hipError_t launch_powm_latency(int lanes, int digits, const PowmArgs &args, hipS
 
 struct VariantEntry {
   PowmVariant v;
-  void (*kernel)(PowmArgs);   // null: launched by launch_powm_latency
+  void (*kernel)(PowmArgs);      // null: launched by launch_powm_latency
+  // The quotient-free form, compiled for the layouts whose classes have the
+  // D spare bits and where it measured faster: 3-5 % for 1024-bit moduli on
+  // 16x3, 8x5 and 4x10.  4x37's form spilled 72 bytes per lane against 32
+  // and ran 0.9 % slower, so 4096-bit moduli keep the quotient.
+  void (*qf_kernel)(PowmArgs);
 };
 
-#define XMP_VARIANT(L, K, D) {{L, K, D}, powm_kernel<L, K, D>}
-#define XMP_LATENCY_VARIANT(L, K, D) {{L, K, D}, nullptr}
+#define XMP_VARIANT(L, K, D) {{L, K, D}, powm_kernel<L, K, D>, nullptr}
+#define XMP_VARIANT_QF(L, K, D) {{L, K, D}, powm_kernel<L, K, D>, powm_kernel<L, K, D, true>}
+#define XMP_LATENCY_VARIANT(L, K, D) {{L, K, D}, nullptr, nullptr}
 const VariantEntry kVariants[] = {
     // <=1024-bit: one lane per instance
     XMP_VARIANT(1, 12, 28),  // 336-bit R : up to 320-bit moduli
-    XMP_VARIANT(1, 20, 28),  // 560 : 512-bit
+    XMP_VARIANT_QF(1, 20, 28),  // 560 : 512-bit
     XMP_VARIANT(1, 28, 28),  // 784 : 768-bit
     XMP_VARIANT(1, 37, 28),  // 1036 : 1024-bit
-    XMP_VARIANT(2, 19, 28),  // 1064 : 1024-bit on two lanes
-    XMP_VARIANT(2, 28, 28),  // 1568 : 1536-bit
+    XMP_VARIANT_QF(2, 19, 28),  // 1064 : 1024-bit on two lanes
+    XMP_VARIANT_QF(2, 28, 28),  // 1568 : 1536-bit
     XMP_VARIANT(2, 37, 28),  // 2072 : 2048-bit
-    XMP_VARIANT(4, 19, 28),  // 2128 : 2048-bit on four lanes
-    XMP_VARIANT(4, 28, 28),  // 3136 : 3072-bit
+    XMP_VARIANT_QF(4, 19, 28),  // 2128 : 2048-bit on four lanes
+    XMP_VARIANT_QF(4, 28, 28),  // 3136 : 3072-bit
     XMP_VARIANT(2, 55, 28),  // 3080 : 3072-bit on two lanes
     XMP_VARIANT(4, 38, 27),  // 4104 : 4096-bit
     XMP_VARIANT(8, 19, 27),  // 4104 : 4096-bit on eight lanes
-    XMP_VARIANT(8, 29, 27),  // 6264 : 6144-bit
+    XMP_VARIANT_QF(8, 29, 27),  // 6264 : 6144-bit
     XMP_VARIANT(8, 38, 27),  // 8208 : 8192-bit
     XMP_VARIANT(16, 19, 27), // 8208 : 8192-bit on sixteen lanes (one DPP row)
     // small moduli, one lane per instance
     XMP_VARIANT(1, 5, 28),   // 140 : 128-bit
     XMP_VARIANT(1, 10, 28),  // 280 : 256-bit
     // above 8192 bits: half a wave / a whole wave per instance, radix 2^26
-    XMP_VARIANT(32, 20, 26), // 16640 : 16384-bit
-    XMP_VARIANT(64, 20, 26), // 33280 : 32768-bit
+    XMP_VARIANT_QF(32, 20, 26), // 16640 : 16384-bit
+    XMP_VARIANT_QF(64, 20, 26), // 33280 : 32768-bit
     // low-latency layouts (fewer digits per lane, so one wave finishes sooner)
     XMP_LATENCY_VARIANT(4, 10, 28),  // 1120 : 1024-bit on four lanes
     XMP_LATENCY_VARIANT(8, 5, 28),   // 1120 : 1024-bit on eight lanes
@@ -550,9 +614,10 @@ const VariantEntry kVariants[] = {
     XMP_LATENCY_VARIANT(16, 3, 28),  // 1344 : 1024-bit on sixteen lanes
     // radix 2^28 above 127 digits, with carry passes inside each product
     XMP_VARIANT(4, 37, 28),  // 4144 : 4096-bit
-    XMP_VARIANT(8, 37, 28),  // 8288 : 8192-bit
+    XMP_VARIANT_QF(8, 37, 28),  // 8288 : 8192-bit
 };
 #undef XMP_VARIANT
+#undef XMP_VARIANT_QF
 #undef XMP_LATENCY_VARIANT
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -594,13 +659,7 @@ __global__ void __launch_bounds__(64, 2) r2_square_kernel(R2Args args) {
     mont.m[k] = limb_digit_guarded<D>(args.mod.data, args.mod.stride, mi, args.mod.len, d);
     a[k] = limb_digit_guarded<D>(args.y, args.y_stride, mi, args.mod.len, d);
   }
-  {
-    const uint32_t m0 = ld_limb(args.mod, mi, 0);
-    uint32_t inv = m0;
-#pragma unroll
-    for (int it = 0; it < 4; it++) inv = inv * (2u - m0 * inv);
-    mont.np0 = (0u - inv) & M::MASK;
-  }
+  mont.np0 = neg_inverse<D>(ld_limb(args.mod, mi, 0));
   for (uint32_t s = 0; s < args.squarings; s++) {
 #pragma unroll
     for (int k = 0; k < K; k++) b[l * K + k] = a[k];
@@ -683,14 +742,7 @@ __global__ void __launch_bounds__(64, 2) crt_combine_kernel(CrtArgs args) {
     b[dk] = limb_digit_guarded<D>(args.mp.data, args.mp.stride, xi, args.mp.len, dk);
     d[dk] = limb_digit_guarded<D>(args.mq.data, args.mq.stride, yi, args.mq.len, dk);
   }
-  {
-    // np0 = -n^-1 mod 2^D (Newton)
-    const uint32_t n0 = ld_limb(args.n, ni, 0);
-    uint32_t inv = n0;
-#pragma unroll
-    for (int it = 0; it < 4; it++) inv = inv * (2u - n0 * inv);
-    mont.np0 = (0u - inv) & M::MASK;
-  }
+  mont.np0 = neg_inverse<D>(ld_limb(args.n, ni, 0));
   wave_sync();
   uint64_t t[K];
 #pragma unroll
@@ -859,7 +911,10 @@ hipError_t launch_powm_variant(int variant, const PowmArgs &args, hipStream_t st
   const PowmVariant &v = kVariants[variant].v;
   if (!kVariants[variant].kernel) return launch_powm_latency(v.lanes, v.digits, args, stream);
   const uint32_t per_block = 64u / (uint32_t)v.lanes;
-  hipLaunchKernelGGL(kVariants[variant].kernel, dim3(div_round_up(args.count, per_block)), dim3(64), 0, stream, args);
+  void (*kernel)(PowmArgs) = kVariants[variant].kernel;
+  if (kVariants[variant].qf_kernel && powm_quotient_free(v.lanes, v.digits, v.radix, args.mod.len))
+    kernel = kVariants[variant].qf_kernel;
+  hipLaunchKernelGGL(kernel, dim3(div_round_up(args.count, per_block)), dim3(64), 0, stream, args);
   return hipGetLastError();
 }
 

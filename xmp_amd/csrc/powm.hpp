@@ -57,6 +57,14 @@ __host__ __device__ inline uint32_t powm_active_lanes(uint32_t L, uint32_t K, ui
   const uint32_t nl = (nd + K - 1u) / K;
   return nl < 1u ? 1u : (nl > L ? L : nl);
 }
+// Whether those lanes hold D bits more than R >= 4m needs, so that the
+// Montgomery reduction can run by m~ = m * (-m^-1 mod 2^D) (< 2^(32 m_len +
+// D)) at the same R: m~ is -1 mod 2^D and each step's quotient digit is the
+// low digit itself (the quotient-free form; products stay congruent mod m
+// and the last one reduces by m).
+__host__ __device__ inline bool powm_quotient_free(uint32_t L, uint32_t K, uint32_t D, uint32_t m_len) {
+  return K * D * powm_active_lanes(L, K, D, m_len) >= 32u * m_len + 2u + D;
+}
 // the R exponent (bits) a variant runs with for moduli of m_len limbs
 inline uint32_t variant_active_rbits(const PowmVariant &v, uint32_t m_len) {
   return (uint32_t)(v.digits * v.radix) * powm_active_lanes(v.lanes, v.digits, v.radix, m_len);
