@@ -123,11 +123,7 @@ struct Mont {
   // m~ = m * np0 of the modulus, which is -1 mod 2^D, so q = f mod 2^D.
   template <bool QF>
   __device__ __forceinline__ uint32_t quotient(uint64_t f) const {
-    if constexpr (QF) {
-      uint32_t x = (uint32_t)f;
-      asm("" : "+v"(x));
-      return g.bcast0_and(x, mask);
-    }
+    if constexpr (QF) return g.bcast0_and((uint32_t)f, mask);
     return g.bcast0_and(mul_lo_mad((uint32_t)f, np0), mask);
   }
 
