@@ -570,9 +570,10 @@ struct VariantEntry {
   PowmVariant v;
   void (*kernel)(PowmArgs);      // null: launched by launch_powm_latency
   // The quotient-free form, compiled for the layouts whose classes have the
-  // D spare bits and where it measured faster: 3-5 % for 1024-bit moduli on
-  // 16x3, 8x5 and 4x10.  4x37's form spilled 72 bytes per lane against 32
-  // and ran 0.9 % slower, so 4096-bit moduli keep the quotient.
+  // D spare bits and where it measured faster: 4-7 % for 1024- and 2048-bit
+  // moduli on 16x3, 16x5, 8x5 and 4x10, 1.5 % for 8192-bit ones on 8x37.
+  // 4x37's form spilled 72 bytes per lane against 32 and ran no faster, so
+  // 4096-bit moduli keep the quotient.
   void (*qf_kernel)(PowmArgs);
 };
 
