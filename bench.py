@@ -22,8 +22,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 # The serving loop runs each batch slot on two streams, four with reserved CUs
-# (4 slots: 8 or 16 streams).  HIP maps streams onto this many hardware queues
-# per process (default 4), so raise it to 16 for the streams to run
+# (6 slots: 12 or 24 streams).  HIP maps streams onto this many hardware
+# queues per process (default 4), so raise it to 16 for the streams to run
 # concurrently; it must be set before the first HIP call.
 if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 16:
     os.environ["GPU_MAX_HW_QUEUES"] = "16"
@@ -140,8 +140,14 @@ def run_mixed(h, stream, total, seed, reps=2):
 
 
 # 600,000 requests per rate point: the last batch's drain (20-30 ms) is then a
-# few per cent of the run instead of a tenth at 800 K req/s
-SERVE_SLOTS, SERVE_BATCH, SERVE_REQUESTS = 4, 8192, 600000
+# few per cent of the run instead of a tenth at 800 K req/s.  Six slots of
+# 8,192 (the latency shape) and of 12,288 (the throughput shape), each batch on
+# the layout for its own size: measured on one MI355X against 4-8 slots of
+# 6,144-16,384 and the layouts chosen for all slots in flight, the first held
+# 97 % of the offered rate at 80 % of its 1.31 M req/s saturation, the second
+# saturated highest (1.59 M req/s, p99 24 ms at 800 K req/s).
+SERVE_SLOTS, SERVE_BATCH, SERVE_REQUESTS = 6, 8192, 600000
+SERVE_TP_BATCH = 12288
 
 
 def run_serving(h, nkeys, requests, seed, reserved_cus=0, slots=SERVE_SLOTS, batch=SERVE_BATCH, layout_scale=1):
@@ -460,16 +466,16 @@ def main():
         log("mixed batch: batched %.1f ms, plain %.1f ms (x%.2f)" % (
             mixed["batched_ms"], mixed["plain_ms"], mixed["batched_over_plain"]))
         # ---- serving: 1024 resident keys, Poisson arrivals
-        # the latency-oriented shape (4 slots of 8,192, each batch on the layout
-        # for its own size) gives the p50/p99 figures; the throughput-oriented
-        # shape (8 slots of 16,384, layouts chosen for all slots in flight)
-        # gives the highest saturation
+        # the latency-oriented shape (6 slots of 8,192) gives the p50/p99
+        # figures; the throughput-oriented shape (6 slots of 12,288) the
+        # highest saturation
         serving, serve_sample = run_serving(h, 1024, SERVE_REQUESTS, 99, reserved_cus=args.serve_reserved_cus,
                                             slots=args.serve_slots, batch=args.serve_batch,
                                             layout_scale=args.serve_layout_scale)
         extra["serving"] = serving
         log("serving: %s" % json.dumps(serving))
-        serving_tp, _ = run_serving(h, 1024, SERVE_REQUESTS, 99, slots=8, batch=16384, layout_scale=0)
+        serving_tp, _ = run_serving(h, 1024, SERVE_REQUESTS, 99, slots=SERVE_SLOTS, batch=SERVE_TP_BATCH,
+                                    layout_scale=1)
         extra["serving_throughput_shape"] = serving_tp
         log("serving (throughput shape): %s" % json.dumps(serving_tp))
 
