@@ -30,9 +30,9 @@ PLAN = [
     (256, 256, [4096, 65536]),
     (512, 512, [4096, 65536, 262144]),
     (768, 768, [16384]),
-    (1024, 1024, [1024, 4096, 8192, 16384, 65536]),
+    (1024, 1024, [1024, 2048, 4096, 6144, 8192, 12288, 16384, 32768, 65536]),
     (1536, 1536, [16384]),
-    (2048, 2048, [1024, 4096, 8192, 16384, 65536]),
+    (2048, 2048, [1024, 2048, 4096, 6144, 8192, 12288, 16384, 32768, 65536]),
     (3072, 3072, [4096, 16384]),
     (4096, 4096, [4096, 16384]),
     (6144, 6144, [4096]),
