@@ -15,7 +15,9 @@ namespace xmp {
 // L <= 16 stays inside a 16-lane DPP row), done with DPP moves on the VALU
 // instead of LDS permutes:
 //   bcast0    -- the group's first lane's value to every lane of the group
+//                (bcast0_and: the same, masked, as the Montgomery step uses it)
 //   from_next -- lane l receives lane l+1's value, the group's last lane 0
+//                (next_digit: the Montgomery step's masked digit hand-over)
 //   from_prev -- lane l receives lane l-1's value, the group's first lane 0
 // DPP controls: quad_perm [0,0,2,2] = 0xA0, [0,0,0,0] = 0x00, row_shl:1 =
 // 0x101, row_shr:1 = 0x111, row_shr:4 = 0x114, row_newbcast:0 = 0x150 (lane 0
