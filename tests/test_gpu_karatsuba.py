@@ -1,0 +1,107 @@
+"""GPU: products beyond the row kernels (more than 512 limbs), which split by
+Karatsuba into half-size products of the whole batch, against GMP's mpz_mul
+(the reference test suite's oracle, unit_tests/test.cu:1616,1780).
+
+The cases walk the split's branches: the smallest Karatsuba size (513 limbs,
+h = 257), a multiplier no longer than h (a alone is split), unbalanced
+operands whose sub-products split again, two levels (2048 limbs), truncated
+and zero-padded outputs, all-ones operands (every term and the signed carry
+of the recombination at their extremes), squares, index arrays with a
+broadcast multiplier, in place, and the schoolbook fallback when the scratch
+limit leaves no room for the workspace.
+"""
+import numpy as np
+import pytest
+
+from helpers import O, assert_rows_equal, gmp, inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(seed, n, la, lb):
+    return inputs.uniform(seed, n, la), inputs.uniform(seed + 1, n, lb)
+
+
+@pytest.mark.parametrize("bits_a,bits_b,bits_p", [
+    (16416, 16416, 32832),     # 513 limbs: the first size that splits
+    (16416, 8192, 24608),      # b.len <= h: a alone is split
+    (40032, 30016, 70048),     # unbalanced; a1*b1 splits a alone
+    (65536, 65536, 131072),    # two levels
+    (32768, 32768, 40000),     # truncated product
+    (20000, 20000, 40064),     # output wider than the product
+    (20000, 96, 20096),        # short multiplier
+])
+def test_karatsuba_products(handle, bits_a, bits_b, bits_p):
+    n = 70
+    a, b = _pair(900 + bits_a // 32, n, bits_a // 32, bits_b // 32)
+    A, B = handle.from_array(a), handle.from_array(b)
+    P = handle.integers(bits_p, n)
+    handle.mul(P, A, B)
+    assert_rows_equal(P.get(), gmp(O.GO_MUL, n, bits_p // 32, a, b), "%dx%d->%d" % (bits_a, bits_b, bits_p))
+    # the operands swapped: the longer one is split
+    handle.mul(P, B, A)
+    assert_rows_equal(P.get(), gmp(O.GO_MUL, n, bits_p // 32, a, b), "%dx%d->%d swapped" % (bits_a, bits_b, bits_p))
+
+
+@pytest.mark.parametrize("bits", [16416, 32768, 65536])
+def test_karatsuba_all_ones_and_squares(handle, bits):
+    n, limbs = 67, bits // 32
+    ones = np.full((n, limbs), 0xFFFFFFFF, np.uint32)
+    A = handle.from_array(ones)
+    P = handle.integers(2 * bits, n)
+    handle.mul(P, A, A)   # a == b: the square path
+    want = O.from_int(((1 << bits) - 1) ** 2, 2 * limbs)
+    assert all(np.array_equal(r, want) for r in P.get()), "(2^%d - 1)^2" % bits
+    # (2^k - 1) * (2^k - 2): a product, not a square, at the carry bound
+    twos = ones.copy()
+    twos[:, 0] = 0xFFFFFFFE
+    B = handle.from_array(twos)
+    handle.mul(P, A, B)
+    want = O.from_int(((1 << bits) - 1) * ((1 << bits) - 2), 2 * limbs)
+    assert all(np.array_equal(r, want) for r in P.get()), "(2^%d - 1)(2^%d - 2)" % (bits, bits)
+    a = inputs.uniform(77 + bits, n, limbs)
+    A = handle.from_array(a)
+    handle.mul(P, A, A)
+    assert_rows_equal(P.get(), gmp(O.GO_MUL, n, 2 * limbs, a, a), "random squares, %d bits" % bits)
+
+
+def test_karatsuba_indices_broadcast_and_in_place(handle):
+    bits, n = 20480, 130
+    limbs = bits // 32
+    a = inputs.uniform(301, n, limbs)
+    b = inputs.uniform(302, 3, limbs)
+    rng = np.random.default_rng(7)
+    ia, ip = rng.permutation(n).astype(np.uint32), rng.permutation(n).astype(np.uint32)
+    A, B = handle.from_array(a), handle.from_array(b)
+    P = handle.integers(2 * bits, n)
+    handle.set_indices(0, ip)
+    handle.set_indices(1, ia)
+    try:
+        handle.mul(P, A, B)
+    finally:
+        handle.clear_indices()
+    rows = gmp(O.GO_MUL, n, 2 * limbs, a[ia], b[np.arange(n) % 3])
+    want = np.zeros_like(rows)
+    want[ip] = rows
+    assert_rows_equal(P.get(), want, "indexed Karatsuba product")
+    # in place, truncated to the operand's precision
+    B2 = handle.from_array(inputs.uniform(303, n, limbs))
+    b2 = B2.get()
+    handle.mul(A, A, B2)
+    assert_rows_equal(A.get(), gmp(O.GO_MUL, n, limbs, a, b2), "in-place Karatsuba product")
+
+
+def test_karatsuba_scratch_limit_falls_back(fresh_handle):
+    """A scratch limit below the workspace keeps the schoolbook kernel: the
+    same products."""
+    h = fresh_handle
+    bits, n = 16416, 40
+    a, b = _pair(401, n, bits // 32, bits // 32)
+    A, B = h.from_array(a), h.from_array(b)
+    P = h.integers(2 * bits, n)
+    h.set_scratch_limit(4096)
+    try:
+        h.mul(P, A, B)
+    finally:
+        h.set_scratch_limit(0)
+    assert_rows_equal(P.get(), gmp(O.GO_MUL, n, 2 * bits // 32, a, b), "schoolbook fallback")

@@ -14,6 +14,7 @@
 // product is formed row by row in 64-bit "lazy" accumulators: each partial
 // product is one v_mad_u64_u32 and carries are resolved once per retired
 // digit, because up to 255 products of 56 bits fit in 64 bits.
+#include <algorithm>
 #include <utility>
 
 #include "devmp.hpp"
@@ -553,6 +554,127 @@ hipError_t launch_sqr(OutOperand p, Operand a, uint32_t count, hipStream_t strea
   if (a.len > 16 && a.len <= 32) return launch_mul_cols<37, 37, true>(p, a, a, count, stream);
   if (a.len > 48 && a.len <= 64) return launch_mul_cols<74, 74, true>(p, a, a, count, stream);
   return launch_mul(p, a, a, count, stream);
+}
+
+// ---- Karatsuba above the register kernels --------------------------------
+// The reference has a register-level Karatsuba (regmp/kar.h:24-207) that no
+// instantiated variant enables (powm.cu:67-70).  Here Karatsuba splits the
+// operands of a whole batch beyond the lane-group row kernels (more than 512
+// limbs, where the fallback is the schoolbook with its row accumulator in HBM):
+// with a = a1 B + a0, b = b1 B + b0 and B = 2^(32h), h = ceil(len(a)/2),
+//   a b = z2 B^2 + (z1 - z0 - z2) B + z0,  z0 = a0 b0, z2 = a1 b1,
+//   z1 = (a0 + a1)(b0 + b1),
+// three half-size products of the whole batch (recursively, until they fit
+// the row kernels), two sums and one pass that adds the five shifted terms
+// with a signed carry.  When b has no more than h limbs, a alone is split:
+// a b = (a1 b) B + a0 b.  The terms live in a workspace, instance i in
+// column i of rows of stride round_up(count, 64).
+
+namespace {
+
+struct KarTerm {
+  const uint32_t *data;
+  uint32_t len, shift;   // limbs, and the term's offset in limbs
+  int32_t sign;          // +1 or -1
+};
+struct KarTerms {
+  KarTerm t[5];
+  uint32_t n;
+  uint32_t stride;       // of every term
+};
+
+// p = sum of the shifted terms, truncated to p.len limbs (the sum is the
+// non-negative product, so the carry is back at 0 past its top).  The carry
+// is signed: the arithmetic shift (which clang defines for signed values)
+// floors it.
+__global__ void __launch_bounds__(256) kar_combine_kernel(OutOperand p, KarTerms c, uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint32_t *out = p.data + out_index(p, i);
+  int64_t carry = 0;
+  for (uint32_t k = 0; k < p.len; k++) {
+    int64_t v = carry;
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+      if ((uint32_t)j < c.n && k >= c.t[j].shift && k - c.t[j].shift < c.t[j].len) {
+        const int64_t x = c.t[j].data[(size_t)(k - c.t[j].shift) * c.stride + i];
+        v += c.t[j].sign > 0 ? x : -x;
+      }
+    }
+    out[(size_t)k * p.stride] = (uint32_t)v;
+    carry = v >> 32;
+  }
+}
+
+// workspace limbs per instance column for a product of a_len x b_len limbs
+// (a_len >= b_len): this level's terms, then the largest child's
+size_t kar_limbs(uint32_t a_len, uint32_t b_len) {
+  if (a_len < kKaratsubaMinLimbs) return 0;
+  const uint32_t h = (a_len + 1) / 2;
+  if (b_len <= h) {
+    const size_t own = (size_t)(h + b_len) + (a_len - h + b_len);
+    return own + std::max(kar_limbs(h, b_len), kar_limbs(std::max(a_len - h, b_len), std::min(a_len - h, b_len)));
+  }
+  const size_t own = 2 * (size_t)(h + 1) + 2 * (size_t)h + 2 * (size_t)(h + 1) + (a_len - h) + (b_len - h);
+  return own + std::max({kar_limbs(h, h), kar_limbs(h + 1, h + 1), kar_limbs(a_len - h, b_len - h)});
+}
+
+}  // namespace
+
+size_t mul_karatsuba_words(uint32_t a_len, uint32_t b_len, uint32_t count) {
+  return kar_limbs(std::max(a_len, b_len), std::min(a_len, b_len)) * round_up(count, kStrideAlign);
+}
+
+hipError_t launch_mul_ws(OutOperand p, Operand a, Operand b, uint32_t count, bool square, uint32_t *ws,
+                         hipStream_t stream) {
+  if (a.len < b.len) std::swap(a, b);
+  if (a.len < kKaratsubaMinLimbs) return square ? launch_sqr(p, a, count, stream) : launch_mul(p, a, b, count, stream);
+  const uint32_t st = round_up(count, kStrideAlign);
+  const uint32_t h = (a.len + 1) / 2;
+  auto take = [&](uint32_t limbs) {
+    uint32_t *d = ws;
+    ws += (size_t)limbs * st;
+    return d;
+  };
+  auto in = [&](const uint32_t *d, uint32_t len) { return Operand{d, len, st, count, nullptr, 0}; };
+  auto out = [&](uint32_t *d, uint32_t len) { return OutOperand{d, len, st, nullptr}; };
+  Operand a0 = a, a1 = a;
+  a0.len = h;
+  a1.data = a.data + (size_t)h * a.stride;
+  a1.len = a.len - h;
+  KarTerms c{};
+  c.stride = st;
+  hipError_t e;
+  if (b.len <= h) {
+    uint32_t *z0 = take(h + b.len), *z2 = take(a1.len + b.len);
+    if ((e = launch_mul_ws(out(z0, h + b.len), a0, b, count, false, ws, stream)) != hipSuccess) return e;
+    if ((e = launch_mul_ws(out(z2, a1.len + b.len), a1, b, count, false, ws, stream)) != hipSuccess) return e;
+    c.t[0] = {z0, h + b.len, 0, 1};
+    c.t[1] = {z2, a1.len + b.len, h, 1};
+    c.n = 2;
+  } else {
+    Operand b0 = b, b1 = b;
+    b0.len = h;
+    b1.data = b.data + (size_t)h * b.stride;
+    b1.len = b.len - h;
+    uint32_t *sa = take(h + 1), *sb = take(h + 1), *z0 = take(2 * h), *z1 = take(2 * h + 2);
+    uint32_t *z2 = take(a1.len + b1.len);
+    if ((e = launch_add(out(sa, h + 1), a0, a1, count, stream)) != hipSuccess) return e;
+    if (!square && (e = launch_add(out(sb, h + 1), b0, b1, count, stream)) != hipSuccess) return e;
+    if ((e = launch_mul_ws(out(z0, 2 * h), a0, b0, count, square, ws, stream)) != hipSuccess) return e;
+    if ((e = launch_mul_ws(out(z2, a1.len + b1.len), a1, b1, count, square, ws, stream)) != hipSuccess) return e;
+    if ((e = launch_mul_ws(out(z1, 2 * h + 2), in(sa, h + 1), in(square ? sa : sb, h + 1), count, square, ws,
+                           stream)) != hipSuccess)
+      return e;
+    c.t[0] = {z0, 2 * h, 0, 1};
+    c.t[1] = {z1, 2 * h + 2, h, 1};
+    c.t[2] = {z0, 2 * h, h, -1};
+    c.t[3] = {z2, a1.len + b1.len, h, -1};
+    c.t[4] = {z2, a1.len + b1.len, 2 * h, 1};
+    c.n = 5;
+  }
+  hipLaunchKernelGGL(kar_combine_kernel, dim3(div_round_up(count, 256)), dim3(256), 0, stream, p, c, count);
+  return hipGetLastError();
 }
 
 }  // namespace xmp

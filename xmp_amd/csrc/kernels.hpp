@@ -39,6 +39,14 @@ hipError_t launch_sub(OutOperand d, Operand a, Operand b, uint32_t count, hipStr
 hipError_t launch_mul(OutOperand p, Operand a, Operand b, uint32_t count, hipStream_t stream);
 // p = a^2 (truncated to p.len limbs): the symmetric kernels where a fits them
 hipError_t launch_sqr(OutOperand p, Operand a, uint32_t count, hipStream_t stream);
+// Products whose longer operand has kKaratsubaMinLimbs limbs or more split by
+// Karatsuba into products the row kernels hold; `ws` holds
+// mul_karatsuba_words(a.len, b.len, count) words (0 below the threshold).
+// square: b is a (the sub-products are squares too).
+constexpr uint32_t kKaratsubaMinLimbs = 513;
+size_t mul_karatsuba_words(uint32_t a_len, uint32_t b_len, uint32_t count);
+hipError_t launch_mul_ws(OutOperand p, Operand a, Operand b, uint32_t count, bool square, uint32_t *ws,
+                         hipStream_t stream);
 
 // ---- division (k_div.hip) ----
 // q and/or r may have data == nullptr to skip that output.

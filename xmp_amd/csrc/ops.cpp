@@ -56,9 +56,18 @@ xmpError_t binary_async(Bin op, xmpHandle_t h, xmpIntegers_t o, xmpIntegers_t a,
   XMP_CHECK_HIP();
   bool inplace = (o == a || o == b);
   uint32_t *buffer = o->limbs;
-  if (inplace) {
+  uint32_t *ws = nullptr;
+  // products beyond the row kernels run Karatsuba in a workspace after the
+  // in-place buffer (without one, within the scratch limit, the schoolbook)
+  const size_t kw = op == Bin::Mul ? mul_karatsuba_words(a->nlimbs, b->nlimbs, count) : 0;
+  const size_t obytes = inplace ? (out_bytes(o) + 255) / 256 * 256 : 0;
+  if (kw && set_necessary_scratch(h, obytes + kw * 4) == xmpErrorSuccess) {
+    ws = (uint32_t *)((uint8_t *)h->scratch + obytes);
+  } else if (inplace) {
     e = set_necessary_scratch(h, out_bytes(o));
     if (e != xmpErrorSuccess) return e;
+  }
+  if (inplace) {
     buffer = (uint32_t *)h->scratch;
     // indexed outputs leave unselected instances untouched: start from o
     if (p->indices[0] &&
@@ -73,7 +82,7 @@ xmpError_t binary_async(Bin op, xmpHandle_t h, xmpIntegers_t o, xmpIntegers_t a,
     case Bin::Sub: st = launch_sub(out, A, B, count, h->stream); break;
     case Bin::Mul:
       if (A.len < B.len) std::swap(A, B);   // reference mul.cu:168-175 orders by precision
-      st = launch_mul(out, A, B, count, h->stream);
+      st = ws ? launch_mul_ws(out, A, B, count, false, ws, h->stream) : launch_mul(out, A, B, count, h->stream);
       break;
   }
   if (st != hipSuccess) return xmpErrorCuda;
@@ -113,16 +122,27 @@ xmpError_t XMPAPI xmpIntegersSqrAsync(xmpHandle_t handle, xmpIntegers_t p, const
   if (p->precision < a->precision) return xmpErrorInvalidPrecision;
   bool inplace = (p == a);
   uint32_t *buffer = p->limbs;
-  if (inplace) {
+  uint32_t *ws = nullptr;
+  // as in binary_async: Karatsuba's workspace after the in-place buffer
+  const size_t kw = mul_karatsuba_words(a->nlimbs, a->nlimbs, count);
+  const size_t obytes = inplace ? (out_bytes(p) + 255) / 256 * 256 : 0;
+  if (kw && set_necessary_scratch(handle, obytes + kw * 4) == xmpErrorSuccess) {
+    ws = (uint32_t *)((uint8_t *)handle->scratch + obytes);
+  } else if (inplace) {
     xmpError_t e = set_necessary_scratch(handle, out_bytes(p));
     if (e != xmpErrorSuccess) return e;
+  }
+  if (inplace) {
     buffer = (uint32_t *)handle->scratch;
     if (pol->indices[0] &&
         hipMemcpyAsync(buffer, p->limbs, out_bytes(p), hipMemcpyDeviceToDevice, handle->stream) != hipSuccess)
       return xmpErrorCuda;
   }
   Operand A = make_in(a, pol, 1);
-  if (launch_sqr(make_out(p, buffer, pol, 0), A, count, handle->stream) != hipSuccess) return xmpErrorCuda;
+  const OutOperand P = make_out(p, buffer, pol, 0);
+  const hipError_t st = ws ? launch_mul_ws(P, A, A, count, true, ws, handle->stream)
+                           : launch_sqr(P, A, count, handle->stream);
+  if (st != hipSuccess) return xmpErrorCuda;
   return finish(handle, inplace, p, buffer);
 }
 xmpError_t XMPAPI xmpIntegersMulAsync(xmpHandle_t handle, xmpIntegers_t p, const xmpIntegers_t a,
