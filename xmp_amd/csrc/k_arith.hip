@@ -563,46 +563,96 @@ hipError_t launch_sqr(OutOperand p, Operand a, uint32_t count, hipStream_t strea
 // limbs, where the fallback is the schoolbook with its row accumulator in HBM):
 // with a = a1 B + a0, b = b1 B + b0 and B = 2^(32h), h = ceil(len(a)/2),
 //   a b = z2 B^2 + (z1 - z0 - z2) B + z0,  z0 = a0 b0, z2 = a1 b1,
-//   z1 = (a0 + a1)(b0 + b1),
-// three half-size products of the whole batch (recursively, until they fit
-// the row kernels), two sums and one pass that adds the five shifted terms
-// with a signed carry.  When b has no more than h limbs, a alone is split:
-// a b = (a1 b) B + a0 b.  The terms live in a workspace, instance i in
-// column i of rows of stride round_up(count, 64).
+//   z1 = (a0 + a1)(b0 + b1).
+// The sums have h limbs and a carry bit each, sa = sa' + ca B and
+// sb = sb' + cb B, so z1 = sa' sb' + (ca sb' + cb sa') B + ca cb B^2: the
+// product stays at h limbs (within the row kernels up to 1,024-limb
+// operands) and the carry terms join the others.  Three half-size products
+// of the whole batch (recursively, until they fit the row kernels), two sums
+// and one pass that adds the shifted terms with a signed carry.  When b has
+// no more than h limbs, a alone is split: a b = (a1 b) B + a0 b.  The terms
+// live in a workspace, instance i in column i of rows of stride
+// round_up(count, 64).
 
 namespace {
 
 struct KarTerm {
-  const uint32_t *data;
-  uint32_t len, shift;   // limbs, and the term's offset in limbs
-  int32_t sign;          // +1 or -1
+  const uint32_t *data;     // term limb k of instance i: data[k * stride + column(i)]
+  const uint32_t *factor;   // null, or a row whose limb i (0 or 1) scales the term
+  uint32_t len, shift;      // limbs, and the term's offset in limbs
+  int32_t sign;             // +1 or -1
+  uint32_t stride;
+  // column(i): i for the workspace's terms; for an input operand's half,
+  // the operand's own gather (idx[i % idx_count] % count, or i % count)
+  const uint32_t *idx;
+  uint32_t idx_count, count;
 };
+constexpr int kKarTerms = 8;
 struct KarTerms {
-  KarTerm t[5];
+  KarTerm t[kKarTerms];
   uint32_t n;
-  uint32_t stride;       // of every term
 };
 
-// p = sum of the shifted terms, truncated to p.len limbs (the sum is the
-// non-negative product, so the carry is back at 0 past its top).  The carry
-// is signed: the arithmetic shift (which clang defines for signed values)
-// floors it.
-__global__ void __launch_bounds__(256) kar_combine_kernel(OutOperand p, KarTerms c, uint32_t count) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  uint32_t *out = p.data + out_index(p, i);
+__device__ __forceinline__ uint32_t kar_column(const KarTerm &t, uint32_t i) {
+  if (t.idx) return t.idx[i % t.idx_count] % t.count;
+  return t.count ? i % t.count : i;
+}
+
+// p = sum of the shifted terms, truncated to p.len limbs.  A workgroup takes
+// 64 instances (a wave's lanes, so every access is a coalesced row) and
+// splits their limbs into kCombineWaves chunks, one wave each, so that a
+// small batch still fills the GPU.  Each wave sums its chunk with a signed
+// carry (the arithmetic shift, which clang defines for signed values, floors
+// it) and leaves its carry-out in LDS; then the carries ripple from chunk to
+// chunk, a round at a time, until no chunk has one left to pass on (almost
+// always after the first round: a carry crosses a chunk only if all of its
+// limbs are ones, or all zeros for a borrow).  The carry past the top limb is
+// dropped (the sum is the non-negative product, truncated).
+constexpr int kCombineWaves = 16;
+__global__ void __launch_bounds__(64 * kCombineWaves) kar_combine_kernel(OutOperand p, KarTerms c, uint32_t count,
+                                                                        uint32_t chunk) {
+  __shared__ int32_t co[kCombineWaves][64];
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  const uint32_t i = blockIdx.x * 64u + lane;
+  const bool act = i < count;
+  const uint32_t ii = act ? i : count - 1;   // a lane past the batch mirrors a valid one and stores nothing
+  const uint32_t k0 = min(w * chunk, p.len), k1 = min(k0 + chunk, p.len);
+  const bool top = k1 == p.len;               // this chunk's carry-out leaves the product
+  uint32_t *out = p.data + out_index(p, ii);
+  uint32_t f[kKarTerms], col[kKarTerms];
+#pragma unroll
+  for (int j = 0; j < kKarTerms; j++) {
+    f[j] = ((uint32_t)j < c.n && c.t[j].factor) ? c.t[j].factor[ii] : 1u;
+    col[j] = (uint32_t)j < c.n ? kar_column(c.t[j], ii) : 0u;
+  }
   int64_t carry = 0;
-  for (uint32_t k = 0; k < p.len; k++) {
+#pragma unroll 4
+  for (uint32_t k = k0; k < k1; k++) {
     int64_t v = carry;
 #pragma unroll
-    for (int j = 0; j < 5; j++) {
-      if ((uint32_t)j < c.n && k >= c.t[j].shift && k - c.t[j].shift < c.t[j].len) {
-        const int64_t x = c.t[j].data[(size_t)(k - c.t[j].shift) * c.stride + i];
+    for (int j = 0; j < kKarTerms; j++) {
+      if ((uint32_t)j < c.n && f[j] && k >= c.t[j].shift && k - c.t[j].shift < c.t[j].len) {
+        const int64_t x = c.t[j].data[(size_t)(k - c.t[j].shift) * c.t[j].stride + col[j]];
         v += c.t[j].sign > 0 ? x : -x;
       }
     }
-    out[(size_t)k * p.stride] = (uint32_t)v;
+    if (act) out[(size_t)k * p.stride] = (uint32_t)v;
     carry = v >> 32;
+  }
+  co[w][lane] = (act && !top) ? (int32_t)carry : 0;
+  __syncthreads();
+  for (int round = 0; round < kCombineWaves; round++) {
+    int64_t cin = w > 0 ? co[w - 1][lane] : 0;
+    __syncthreads();   // every chunk has read its carry-in before any is rewritten
+    if (cin != 0) {
+      for (uint32_t k = k0; k < k1 && cin != 0; k++) {
+        const int64_t x = (int64_t)out[(size_t)k * p.stride] + cin;
+        out[(size_t)k * p.stride] = (uint32_t)x;
+        cin = x >> 32;
+      }
+    }
+    co[w][lane] = top ? 0 : (int32_t)cin;
+    if (!__syncthreads_or(co[w][lane] != 0)) break;
   }
 }
 
@@ -615,8 +665,8 @@ size_t kar_limbs(uint32_t a_len, uint32_t b_len) {
     const size_t own = (size_t)(h + b_len) + (a_len - h + b_len);
     return own + std::max(kar_limbs(h, b_len), kar_limbs(std::max(a_len - h, b_len), std::min(a_len - h, b_len)));
   }
-  const size_t own = 2 * (size_t)(h + 1) + 2 * (size_t)h + 2 * (size_t)(h + 1) + (a_len - h) + (b_len - h);
-  return own + std::max({kar_limbs(h, h), kar_limbs(h + 1, h + 1), kar_limbs(a_len - h, b_len - h)});
+  const size_t own = 2 * (size_t)(h + 1) + 2 * (size_t)h + 2 * (size_t)h + (a_len - h) + (b_len - h);
+  return own + std::max(kar_limbs(h, h), kar_limbs(a_len - h, b_len - h));
 }
 
 }  // namespace
@@ -642,39 +692,62 @@ hipError_t launch_mul_ws(OutOperand p, Operand a, Operand b, uint32_t count, boo
   a0.len = h;
   a1.data = a.data + (size_t)h * a.stride;
   a1.len = a.len - h;
+  // a workspace term, and an input operand's half with its own gather
+  auto term = [&](const uint32_t *d, const uint32_t *factor, uint32_t len, uint32_t shift, int32_t sign) {
+    return KarTerm{d, factor, len, shift, sign, st, nullptr, 0, 0};
+  };
+  auto half = [&](const Operand &o) { return KarTerm{o.data, nullptr, o.len, 0, 1, o.stride, o.idx, o.idx_count, o.count}; };
+  auto combine = [&](OutOperand dst, const KarTerms &c) {
+    const uint32_t chunk = div_round_up(dst.len, (uint32_t)kCombineWaves);
+    hipLaunchKernelGGL(kar_combine_kernel, dim3(div_round_up(count, 64)), dim3(64 * kCombineWaves), 0, stream, dst, c,
+                       count, chunk);
+    return hipGetLastError();
+  };
   KarTerms c{};
-  c.stride = st;
   hipError_t e;
   if (b.len <= h) {
     uint32_t *z0 = take(h + b.len), *z2 = take(a1.len + b.len);
     if ((e = launch_mul_ws(out(z0, h + b.len), a0, b, count, false, ws, stream)) != hipSuccess) return e;
     if ((e = launch_mul_ws(out(z2, a1.len + b.len), a1, b, count, false, ws, stream)) != hipSuccess) return e;
-    c.t[0] = {z0, h + b.len, 0, 1};
-    c.t[1] = {z2, a1.len + b.len, h, 1};
+    c.t[0] = term(z0, nullptr, h + b.len, 0, 1);
+    c.t[1] = term(z2, nullptr, a1.len + b.len, h, 1);
     c.n = 2;
   } else {
     Operand b0 = b, b1 = b;
     b0.len = h;
     b1.data = b.data + (size_t)h * b.stride;
     b1.len = b.len - h;
-    uint32_t *sa = take(h + 1), *sb = take(h + 1), *z0 = take(2 * h), *z1 = take(2 * h + 2);
-    uint32_t *z2 = take(a1.len + b1.len);
-    if ((e = launch_add(out(sa, h + 1), a0, a1, count, stream)) != hipSuccess) return e;
-    if (!square && (e = launch_add(out(sb, h + 1), b0, b1, count, stream)) != hipSuccess) return e;
+    // the sums with their carry limb (row h), then the three products
+    uint32_t *sa = take(h + 1), *sbuf = take(h + 1);
+    uint32_t *sb = square ? sa : sbuf;
+    uint32_t *z0 = take(2 * h), *z1 = take(2 * h), *z2 = take(a1.len + b1.len);
+    const uint32_t *ca = sa + (size_t)h * st, *cb = sb + (size_t)h * st;
+    // the sums on the combine kernel too: its chunks fill the GPU where the
+    // add kernel's one instance per lane would leave a small batch latency-bound
+    KarTerms s{};
+    s.t[0] = half(a0);
+    s.t[1] = half(a1);
+    s.n = 2;
+    if ((e = combine(out(sa, h + 1), s)) != hipSuccess) return e;
+    if (!square) {
+      s.t[0] = half(b0);
+      s.t[1] = half(b1);
+      if ((e = combine(out(sb, h + 1), s)) != hipSuccess) return e;
+    }
     if ((e = launch_mul_ws(out(z0, 2 * h), a0, b0, count, square, ws, stream)) != hipSuccess) return e;
     if ((e = launch_mul_ws(out(z2, a1.len + b1.len), a1, b1, count, square, ws, stream)) != hipSuccess) return e;
-    if ((e = launch_mul_ws(out(z1, 2 * h + 2), in(sa, h + 1), in(square ? sa : sb, h + 1), count, square, ws,
-                           stream)) != hipSuccess)
-      return e;
-    c.t[0] = {z0, 2 * h, 0, 1};
-    c.t[1] = {z1, 2 * h + 2, h, 1};
-    c.t[2] = {z0, 2 * h, h, -1};
-    c.t[3] = {z2, a1.len + b1.len, h, -1};
-    c.t[4] = {z2, a1.len + b1.len, 2 * h, 1};
-    c.n = 5;
+    if ((e = launch_mul_ws(out(z1, 2 * h), in(sa, h), in(sb, h), count, square, ws, stream)) != hipSuccess) return e;
+    c.t[0] = term(z0, nullptr, 2 * h, 0, 1);
+    c.t[1] = term(z1, nullptr, 2 * h, h, 1);
+    c.t[2] = term(sb, ca, h, 2 * h, 1);    // ca sb' B
+    c.t[3] = term(sa, cb, h, 2 * h, 1);    // cb sa' B
+    c.t[4] = term(ca, cb, 1, 3 * h, 1);    // ca cb B^2
+    c.t[5] = term(z0, nullptr, 2 * h, h, -1);
+    c.t[6] = term(z2, nullptr, a1.len + b1.len, h, -1);
+    c.t[7] = term(z2, nullptr, a1.len + b1.len, 2 * h, 1);
+    c.n = 8;
   }
-  hipLaunchKernelGGL(kar_combine_kernel, dim3(div_round_up(count, 256)), dim3(256), 0, stream, p, c, count);
-  return hipGetLastError();
+  return combine(p, c);
 }
 
 }  // namespace xmp
