@@ -431,6 +431,19 @@ def main():
                      ("mul_2048x2048", "GO_MUL", n2, 128, a2, b2, None)]
         del A2, B2, S2, P2
 
+        # ---- mul beyond the row kernels: 4,096 x 32768x32768 bits (Karatsuba)
+        n5 = 4096
+        a5, b5 = inputs.uniform(5151, n5, 1024), inputs.uniform(5152, n5, 1024)
+        A5, B5, P5 = h.from_array(a5), h.from_array(b5), h.integers(65536, n5)
+        mul5 = lambda: lib.xmpIntegersMulAsync(h.h, P5.x, A5.x, B5.x, n5)  # noqa: E731
+        mul5()
+        hip.stream_sync(stream)
+        mul5_ms = time_async(mul5, stream, 5)
+        extra["mul_32768x32768"] = {"pairs": n5, "ms": round(mul5_ms, 4), "mul_per_s": round(n5 / mul5_ms * 1e3, 1)}
+        log("mul 32768x32768 x4096: %.3f ms" % mul5_ms)
+        cpu_work += [("mul_32768x32768", "GO_MUL", n5, 2048, a5, b5, None)]
+        del A5, B5, P5
+
         # ---- div/mod: 65,536 x (4096-bit / 2048-bit, divisor top bit set)
         a3 = inputs.uniform(4343, N, 128)
         d3 = inputs.uniform(4344, N, 64)
