@@ -658,6 +658,177 @@ __global__ void __launch_bounds__(256) divmod_mem_kernel(OutOperand q, OutOperan
   divide_mem(A, b.data, b.stride, bi, b.len, o, scratch + t, scr_stride);
 }
 
+// ---- one wave per division: divisors beyond the register kernels ---------
+// Knuth D as divide_mem (the normalised divisor v, the normalised dividend U
+// divided in place, the window U[j .. j+NB] for quotient digit j), with v and
+// U in LDS and each digit's multiply-subtract split over the wave's 64 lanes:
+// lane l takes the C = ceil((NB+1)/64) window limbs from l*C, with the low and
+// high product words on two borrow chains as in the pair kernel (the high
+// word entering its first limb, hi(q*v[l*C-1]), it computes itself), and
+// hands its outgoing borrows (0..2) to lane l+1, which subtracts them from
+// its first limb.  A borrow crosses a whole chunk only when its limbs are all
+// zero, so the rounds of hand-over end after one almost always (a wave
+// ballot decides).  Every lane computes the same estimate from the window's
+// top limbs (broadcast LDS reads).  With the borrows in, the window's top
+// limb is 0, or all ones when the estimate was one too large (the difference
+// is then in (-V, 0)): V is added back the same way.  One wave per division
+// fills the GPU where the workspace kernel's one lane per division gave
+// 16,384 divisions 256 waves.  The LDS operations of one wave complete in
+// program order, so a wavefront fence between the steps keeps them ordered.
+constexpr uint32_t kWaveDivMaxWords = 16384;   // v and U in 64 KiB of LDS
+
+__device__ __forceinline__ void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// hand each lane's outgoing borrow (or carry, ADD) to the next lane, a round
+// at a time, until none is left below the window's top limb (lane topl, whose
+// outgoing borrow or carry leaves the window)
+template <bool ADD>
+__device__ __forceinline__ void wave_ripple(uint32_t *W, uint32_t k0, uint32_t k1, uint32_t out, uint32_t l,
+                                            uint32_t topl) {
+  if (l >= topl) out = 0;
+  while (__builtin_amdgcn_ballot_w64(out != 0) != 0) {
+    uint32_t cin = (uint32_t)__shfl_up((int)out, 1, 64);
+    if (l == 0) cin = 0;
+    for (uint32_t k = k0; k < k1 && cin != 0; k++) {
+      if (ADD) {
+        const uint64_t x = (uint64_t)W[k] + cin;
+        W[k] = (uint32_t)x;
+        cin = (uint32_t)(x >> 32);
+      } else {
+        const int64_t x = (int64_t)W[k] - (int64_t)cin;
+        W[k] = (uint32_t)x;
+        cin = x < 0 ? 1u : 0u;
+      }
+    }
+    out = l >= topl ? 0u : cin;
+  }
+}
+
+// LDS words divide_wave needs: v, then U (at most la + NB limbs)
+__host__ __device__ inline uint32_t wave_div_words(uint32_t la, uint32_t nb) { return 2 * nb + la; }
+
+template <class Dividend>
+__device__ void divide_wave(const Dividend &A, const uint32_t *bdata, uint32_t bstride, uint32_t binst, uint32_t NB,
+                            const DivOut &o, uint32_t *lds) {
+  const uint32_t l = threadIdx.x & 63u;
+  int top = -1;
+  for (uint32_t k = l; k < NB; k += 64)
+    if (bdata[(size_t)k * bstride + binst]) top = (int)k;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) top = max(top, __shfl_xor(top, d, 64));
+  const uint32_t la = A.length();
+  const int sl = top < 0 ? 0 : (int)NB - 1 - top;
+  const int32_t lu = (int32_t)la + sl + 1;
+  const int32_t nq = lu - (int32_t)NB;
+  if (top < 0 || nq <= 0) {
+    // division by zero, or a dividend shorter than the normalised divisor: q = 0, r = a
+    if (o.q)
+      for (uint32_t k = l; k < o.q_len; k += 64) o.q[(size_t)k * o.q_stride + o.q_inst] = 0u;
+    if (o.r)
+      for (uint32_t k = l; k < o.r_len; k += 64) o.r[(size_t)k * o.r_stride + o.r_inst] = A((int32_t)k);
+    return;
+  }
+  const uint32_t s = (uint32_t)__builtin_clz(bdata[(size_t)top * bstride + binst]);
+  uint32_t *v = lds;        // NB limbs
+  uint32_t *U = lds + NB;   // lu <= la + NB limbs
+  for (uint32_t k = l; k < NB; k += 64) {
+    const int src = (int)k - sl;
+    const uint32_t hi = src >= 0 ? bdata[(size_t)src * bstride + binst] : 0u;
+    const uint32_t lo = src >= 1 ? bdata[(size_t)(src - 1) * bstride + binst] : 0u;
+    v[k] = shl_limb(hi, lo, s);
+  }
+  for (int32_t k = (int32_t)l; k < lu; k += 64) U[k] = shl_limb(A(k - sl), A(k - sl - 1), s);
+  wave_fence();
+  const uint32_t vt = v[NB - 1], vt2 = v[NB - 2];
+  const uint32_t dinv = reciprocal(vt);
+  const uint32_t C = (NB + 1 + 63) / 64;
+  const uint32_t k0 = min(l * C, NB + 1), k1 = min(k0 + C, NB + 1);
+  const uint32_t topl = NB / C;   // the lane holding the window's top limb
+  for (int32_t j = nq - 1; j >= 0; j--) {
+    uint32_t *W = U + j;
+    const uint32_t wt = W[NB], wt1 = W[NB - 1], wt2 = W[NB - 2];
+    uint32_t qhat, rhat;
+    bool rhat_ok = true;
+    if (wt >= vt) {
+      qhat = 0xFFFFFFFFu;
+      const uint64_t rr = (uint64_t)wt1 + vt;
+      rhat = (uint32_t)rr;
+      rhat_ok = (rr >> 32) == 0;
+    } else {
+      qhat = div_2by1(wt, wt1, vt, dinv, rhat);
+    }
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+      if (rhat_ok && (uint64_t)qhat * vt2 > (((uint64_t)rhat << 32) | wt2)) {
+        qhat--;
+        const uint64_t rr = (uint64_t)rhat + vt;
+        rhat = (uint32_t)rr;
+        rhat_ok = (rr >> 32) == 0;
+      }
+    }
+    // W - qhat*V over this lane's limbs: limb k loses lo(q*v[k]) and hi(q*v[k-1])
+    uint32_t b1 = 0, b2 = 0;
+    uint32_t ph = (k0 >= 1 && k0 - 1 < NB) ? (uint32_t)(((uint64_t)qhat * v[k0 - 1]) >> 32) : 0u;
+    for (uint32_t k = k0; k < k1; k++) {
+      const uint64_t p = k < NB ? (uint64_t)qhat * v[k] : 0ull;
+      const uint32_t x = subb(W[k], (uint32_t)p, b1, b1);
+      W[k] = subb(x, ph, b2, b2);
+      ph = (uint32_t)(p >> 32);
+    }
+    wave_ripple<false>(W, k0, k1, b1 + b2, l, topl);
+    wave_fence();
+    if (W[NB] != 0u) {
+      // the estimate was one too large (rare; the same on every lane): add V back
+      qhat--;
+      uint32_t c = 0;
+      for (uint32_t k = k0; k < k1; k++) W[k] = addc(W[k], k < NB ? v[k] : 0u, c, c);
+      wave_ripple<true>(W, k0, k1, c, l, topl);
+      wave_fence();
+    }
+    if (o.q && l == 0 && (uint32_t)j < o.q_len) o.q[(size_t)j * o.q_stride + o.q_inst] = qhat;
+  }
+  if (o.q)
+    for (uint32_t k = (uint32_t)nq + l; k < o.q_len; k += 64) o.q[(size_t)k * o.q_stride + o.q_inst] = 0u;
+  if (o.r) {
+    // remainder limb k: U[k+sl .. k+sl+1] >> s, within U's NB low limbs
+    for (uint32_t k = l; k < o.r_len; k += 64) {
+      const uint32_t kk = k + (uint32_t)sl;
+      const uint32_t lo = kk < NB ? U[kk] : 0u, hi = kk + 1 < NB ? U[kk + 1] : 0u;
+      o.r[(size_t)k * o.r_stride + o.r_inst] = s ? funnel(hi, lo, s) : lo;
+    }
+  }
+}
+
+// one 64-lane workgroup per division, v and U in dynamic LDS
+__global__ void __launch_bounds__(64) divmod_wave_kernel(OutOperand q, OutOperand r, Operand a, Operand b) {
+  extern __shared__ uint32_t lds_words[];
+  const uint32_t t = blockIdx.x;
+  DivOut o;
+  o.q = q.data;
+  o.q_len = q.len;
+  o.q_stride = q.stride;
+  o.q_inst = q.data ? out_index(q, t) : 0;
+  o.r = r.data;
+  o.r_len = r.len;
+  o.r_stride = r.stride;
+  o.r_inst = r.data ? out_index(r, t) : 0;
+  OperandDividend A{a.data, a.stride, in_index(a, t), a.len};
+  divide_wave(A, b.data, b.stride, in_index(b, t), b.len, o, lds_words);
+}
+
+__global__ void __launch_bounds__(64) pow2_mod_wave_kernel(uint32_t *out, uint32_t out_len, uint32_t out_stride,
+                                                           const uint32_t *m, uint32_t m_len, uint32_t m_stride,
+                                                           uint32_t e) {
+  extern __shared__ uint32_t lds_words[];
+  const uint32_t t = blockIdx.x;
+  DivOut o{nullptr, 0, 0, 0, out, out_len, out_stride, t};
+  PowerOfTwoDividend A{e};
+  divide_wave(A, m, m_stride, t, m_len, o, lds_words);
+}
+
 __global__ void __launch_bounds__(256) pow2_mod_mem_kernel(uint32_t *out, uint32_t out_len, uint32_t out_stride,
                                                            const uint32_t *m, uint32_t m_len, uint32_t m_stride,
                                                            uint32_t count, uint32_t e, uint32_t *scratch,
@@ -718,6 +889,11 @@ hipError_t launch_divmod_ws(OutOperand q, OutOperand r, Operand a, Operand b, ui
   if (nb <= 64) return launch_divmod_reg<64>(q, r, a, b, count, stream);
   if (nb <= 96) return launch_divmod_reg<96>(q, r, a, b, count, stream);
   if (nb <= 128) return launch_divmod_reg<128>(q, r, a, b, count, stream);
+  const uint32_t words = wave_div_words(a.len, nb);
+  if (words <= kWaveDivMaxWords) {
+    hipLaunchKernelGGL(divmod_wave_kernel, dim3(count), dim3(64), words * sizeof(uint32_t), stream, q, r, a, b);
+    return hipGetLastError();
+  }
   if (!scratch) return hipErrorInvalidValue;
   hipLaunchKernelGGL(divmod_mem_kernel, dim3(div_round_up(count, 256)), dim3(256), 0, stream, q, r, a, b, count,
                      scratch, round_up(count, 256));
@@ -732,6 +908,12 @@ hipError_t launch_pow2_mod(uint32_t *out, uint32_t out_len, uint32_t out_stride,
   if (m_len <= 64) return launch_pow2_reg<64>(out, out_len, out_stride, m, m_len, m_stride, count, e, stream);
   if (m_len <= 96) return launch_pow2_reg<96>(out, out_len, out_stride, m, m_len, m_stride, count, e, stream);
   if (m_len <= 128) return launch_pow2_reg<128>(out, out_len, out_stride, m, m_len, m_stride, count, e, stream);
+  const uint32_t words = wave_div_words((e >> 5) + 1, m_len);
+  if (words <= kWaveDivMaxWords) {
+    hipLaunchKernelGGL(pow2_mod_wave_kernel, dim3(count), dim3(64), words * sizeof(uint32_t), stream, out, out_len,
+                       out_stride, m, m_len, m_stride, e);
+    return hipGetLastError();
+  }
   if (!scratch) return hipErrorInvalidValue;
   hipLaunchKernelGGL(pow2_mod_mem_kernel, dim3(div_round_up(count, 256)), dim3(256), 0, stream, out, out_len,
                      out_stride, m, m_len, m_stride, count, e, scratch, round_up(count, 256));
