@@ -677,6 +677,7 @@ size_t mul_karatsuba_words(uint32_t a_len, uint32_t b_len, uint32_t count) {
 
 hipError_t launch_mul_ws(OutOperand p, Operand a, Operand b, uint32_t count, bool square, uint32_t *ws,
                          hipStream_t stream) {
+  if (count == 0) return hipSuccess;
   if (a.len < b.len) std::swap(a, b);
   if (a.len < kKaratsubaMinLimbs) return square ? launch_sqr(p, a, count, stream) : launch_mul(p, a, b, count, stream);
   const uint32_t st = round_up(count, kStrideAlign);

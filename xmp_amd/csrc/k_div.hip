@@ -870,6 +870,7 @@ hipError_t launch_divmod_pair(OutOperand q, OutOperand r, Operand a, Operand b, 
 
 hipError_t launch_divmod_ws(OutOperand q, OutOperand r, Operand a, Operand b, uint32_t count, uint32_t *scratch,
                             hipStream_t stream) {
+  if (count == 0) return hipSuccess;
   const uint32_t nb = b.len;
   // Below 131,072 divisions one lane each gives fewer than two waves per
   // SIMD (1,024 SIMDs x 2 x 64 lanes): two lanes per division instead
@@ -902,6 +903,7 @@ hipError_t launch_divmod_ws(OutOperand q, OutOperand r, Operand a, Operand b, ui
 
 hipError_t launch_pow2_mod(uint32_t *out, uint32_t out_len, uint32_t out_stride, const uint32_t *m, uint32_t m_len,
                            uint32_t m_stride, uint32_t count, uint32_t e, uint32_t *scratch, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
   if (m_len <= 8) return launch_pow2_reg<8>(out, out_len, out_stride, m, m_len, m_stride, count, e, stream);
   if (m_len <= 16) return launch_pow2_reg<16>(out, out_len, out_stride, m, m_len, m_stride, count, e, stream);
   if (m_len <= 32) return launch_pow2_reg<32>(out, out_len, out_stride, m, m_len, m_stride, count, e, stream);
