@@ -48,9 +48,7 @@ size_t mul_karatsuba_words(uint32_t a_len, uint32_t b_len, uint32_t count);
 hipError_t launch_mul_ws(OutOperand p, Operand a, Operand b, uint32_t count, bool square, uint32_t *ws,
                          hipStream_t stream);
 
-// ---- division (k_div.hip) ----
-// q and/or r may have data == nullptr to skip that output.
-hipError_t launch_divmod(OutOperand q, OutOperand r, Operand a, Operand b, uint32_t count, hipStream_t stream);
+// ---- division (k_div.hip): launch_divmod_ws and launch_pow2_mod in powm.hpp ----
 
 // ---- bitwise (k_logic.hip), reference src/logic.cu ----
 enum : int { kBitIor = 0, kBitAnd = 1, kBitXor = 2, kBitNot = 3 };
