@@ -139,14 +139,15 @@ def run_mixed(h, stream, total, seed, reps=2):
             "batched_over_plain": round(plain_ms / batched_ms, 3)}, samples
 
 
-# 600,000 requests per rate point: the last batch's drain (20-30 ms) is then a
-# few per cent of the run instead of a tenth at 800 K req/s.  Six slots of
+# 1,200,000 requests per rate point: the last batches' drain (20-40 ms), which
+# the achieved rate counts in the run's wall time, is then 2-3 % of a run at
+# 800 K-1 M req/s (it was 4-5 % with 600,000).  Six slots of
 # 8,192 (the latency shape) and of 12,288 (the throughput shape), each batch on
 # the layout for its own size: measured on one MI355X against 4-8 slots of
 # 6,144-16,384 and the layouts chosen for all slots in flight, the first held
 # 97 % of the offered rate at 80 % of its 1.31 M req/s saturation, the second
 # saturated highest (1.59 M req/s, p99 24 ms at 800 K req/s).
-SERVE_SLOTS, SERVE_BATCH, SERVE_REQUESTS = 6, 8192, 600000
+SERVE_SLOTS, SERVE_BATCH, SERVE_REQUESTS = 6, 8192, 1200000
 SERVE_TP_BATCH = 12288
 
 
