@@ -6,7 +6,9 @@ The cases walk the split's branches: the smallest Karatsuba size (513 limbs,
 h = 257), a multiplier no longer than h (a alone is split), unbalanced
 operands whose sub-products split again, two levels (2048 limbs), truncated
 and zero-padded outputs, all-ones operands (every term and the signed carry
-of the recombination at their extremes), squares, index arrays with a
+of the recombination at their extremes), the one term that reaches past the
+product (ca cb B^2 at limb 3h = len(a) + len(b)) with the product's end on a
+chunk boundary of the recombination, squares, index arrays with a
 broadcast multiplier, in place, and the schoolbook fallback when the scratch
 limit leaves no room for the workspace.
 """
@@ -63,6 +65,23 @@ def test_karatsuba_all_ones_and_squares(handle, bits):
     A = handle.from_array(a)
     handle.mul(P, A, A)
     assert_rows_equal(P.get(), gmp(O.GO_MUL, n, 2 * limbs, a, a), "random squares, %d bits" % bits)
+
+
+def test_karatsuba_carry_term_past_the_product(handle):
+    """1087 x 545 limbs: h = 544, len(a) + len(b) = 3h = 1632 (51 chunks of
+    32 limbs), so the product of the half sums' carry limbs sits at limb
+    1632, one past the product, where the carries from below cancel it; the
+    output runs 32 limbs further.  All-ones operands set both carry limbs."""
+    n, la, lb, lp = 66, 1087, 545, 1664
+    ones_a = np.full((n, la), 0xFFFFFFFF, np.uint32)
+    ones_b = np.full((n, lb), 0xFFFFFFFF, np.uint32)
+    P = handle.integers(32 * lp, n)
+    handle.mul(P, handle.from_array(ones_a), handle.from_array(ones_b))
+    want = O.from_int(((1 << (32 * la)) - 1) * ((1 << (32 * lb)) - 1), lp)
+    assert all(np.array_equal(r, want) for r in P.get()), "(2^34784 - 1)(2^17440 - 1)"
+    a, b = _pair(611, n, la, lb)
+    handle.mul(P, handle.from_array(a), handle.from_array(b))
+    assert_rows_equal(P.get(), gmp(O.GO_MUL, n, lp, a, b), "1087 x 545 limbs -> 1664")
 
 
 def test_karatsuba_indices_broadcast_and_in_place(handle):
