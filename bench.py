@@ -214,6 +214,24 @@ def run_only(args):
         ctx = Montgomery(h, M, 2048, 65536)
         fn = lambda: ctx.powm_async(out, B, E, 65536)  # noqa: E731
         what = "powm_kernel 65,536 x 2048-bit (algorithmic bytes: 4 x 256 B in/out per instance)"
+    elif args.only == "powm1024":
+        m1, b1, e1, M1, B1, E1 = powm_workload(h, 4096, 1024, 1024)
+        out1 = h.integers(1024, 4096)
+        fn = lambda: h.powm(out1, B1, E1, M1, 4096, asynchronous=True)  # noqa: E731
+        what = "powm 4,096 x 1024-bit (R^2 mod m and the exponentiation, as xmpIntegersPowm)"
+    elif args.only == "mul32768":
+        a5, b5 = inputs.uniform(5151, 4096, 1024), inputs.uniform(5152, 4096, 1024)
+        A5, B5, P5 = h.from_array(a5), h.from_array(b5), h.integers(65536, 4096)
+        fn = lambda: lib.xmpIntegersMulAsync(h.h, P5.x, A5.x, B5.x, 4096)  # noqa: E731
+        what = "mul 4,096 x 32768x32768 -> 65536-bit (Karatsuba over the row kernels)"
+    elif args.only == "divwide":
+        a6 = inputs.uniform(6161, 16384, 512)
+        d6 = inputs.uniform(6162, 16384, 256)
+        d6[:, -1] |= np.uint32(0x80000000)
+        A6, D6 = h.from_array(a6), h.from_array(d6)
+        Q6, R6 = h.integers(16384, 16384), h.integers(8192, 16384)
+        fn = lambda: lib.xmpIntegersDivModAsync(h.h, Q6.x, R6.x, A6.x, D6.x, 16384)  # noqa: E731
+        what = "divmod_wave_kernel 16,384 x 16384-bit / 8192-bit (one wave per division)"
     elif args.only == "mul":
         n2 = 1 << 20
         a2, b2 = inputs.ordered_pair(4242, n2, 64)
@@ -270,9 +288,12 @@ def main():
     ap.add_argument("--serve-batch", type=int, default=SERVE_BATCH, help="serving requests per slot")
     ap.add_argument("--serve-layout-scale", type=int, default=1,
                     help="serving layout chosen as for a batch this many times larger (0 = slots in flight)")
-    ap.add_argument("--only", choices=["powm", "add", "mul", "sqr", "div", "copy", "mixed", "serve"],
-                    help="run just one kernel (for counter profiling): the 2048-bit powm kernel with resident "
-                         "constants, the 2048-bit add, or a plain 256 MiB import copy for counter calibration")
+    ap.add_argument("--only", choices=["powm", "powm1024", "add", "mul", "mul32768", "sqr", "div", "divwide", "copy",
+                                       "mixed", "serve"],
+                    help="run just one workload (for profiling): the 2048-bit powm kernel with resident "
+                         "constants, 4,096 x 1024-bit powm, the 2048-bit add/mul/square, 32768-bit products, "
+                         "4096/2048 and 16384/8192 division, a plain 256 MiB import copy for counter calibration, "
+                         "the mixed batch or the serving loop")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the instances are independent, so N GPUs would each run this "
