@@ -223,6 +223,29 @@ def test_mul_large_batches(handle, bits_a, bits_b, bits_p, square):
     assert_rows_equal(got[ip[rows]], want, "indexed mul, 400,000 products")
 
 
+@pytest.mark.parametrize("bits,count", [(8192, 4095), (8192, 4096), (16384, 4095), (16384, 4096), (6144, 2048),
+                                        (12288, 2048)])
+def test_mul_group_width_by_batch(handle, bits, count):
+    """Above 4096 bits the lane-group width follows the batch (32 lanes below
+    4,096 products, else 8 or 16): each side of each switch, and operands
+    that fill only part of the group's digits, against GMP on a sample
+    (all-ones rows first)."""
+    limbs = bits // 32
+    a = inputs.uniform(3100 + count, count, limbs)
+    b = inputs.uniform(3200 + count, count, limbs)
+    a[:3] = 0xFFFFFFFF
+    b[:2] = 0xFFFFFFFF
+    A, B = handle.from_array(a), handle.from_array(b)
+    P = handle.integers(2 * bits, count)
+    handle.mul(P, A, B)
+    rows = np.unique(np.concatenate([np.arange(4), np.linspace(0, count - 1, 300).astype(int)]))
+    assert_rows_equal(P.get()[rows], gmp(O.GO_MUL, rows.size, 2 * limbs, a[rows], b[rows]),
+                      "%d bits x %d" % (bits, count))
+    handle.mul(P, A, A)
+    assert_rows_equal(P.get()[rows], gmp(O.GO_MUL, rows.size, 2 * limbs, a[rows], a[rows]),
+                      "%d-bit squares x %d" % (bits, count))
+
+
 def test_mul_closed_form_and_square(handle):
     """(2^k - 1)^2 = 2^2k - 2^(k+1) + 1, and mul(a, a) takes the square path."""
     n = 64
@@ -366,3 +389,61 @@ def test_div_addback_and_both_lane_layouts(handle, nb):
         wq, wr = gmp(O.GO_DIV, rows.size, 2 * nb, a2[rows], b2[rows], out2_limbs=nb)
         assert_rows_equal(Q.get()[rows], wq, "q, %d divisions" % cnt)
         assert_rows_equal(R.get()[rows], wr, "r, %d divisions" % cnt)
+
+
+def _later_digit_cases(nb, n, seed):
+    """Dividends whose later quotient digits sit at the estimate's edges:
+    (1) a window (q0 + 1) T after an exact digit, so that digit's estimate is
+    one too large (add-back); (2) a next window whose top three limbs are one
+    below a multiple of V's top two, so the borrows from the limbs below
+    decide the estimate; (3) divisors of nearly all ones with remainders
+    within a few units of V (the window's top limbs all ones).  Returns a
+    (nb + 3 limbs) and b (nb limbs)."""
+    rng = np.random.default_rng(seed)
+    B = 1 << 32
+
+    def r(bits):
+        return int.from_bytes(rng.bytes((bits + 7) // 8), "little") % (1 << bits) if bits else 0
+
+    a = np.zeros((n, nb + 3), np.uint32)
+    b = np.zeros((n, nb), np.uint32)
+    for i in range(n):
+        kind = i % 3
+        if kind == 0:
+            vt, vt2 = r(32) | 0x80000000, r(32)
+            t = vt * B ** (nb - 1) + vt2 * B ** (nb - 2)
+            v = t + B ** (nb - 2) - 1
+            w2 = (int(rng.integers(1, B - 1)) + 1) * t
+            r1, u0 = divmod(w2, B)
+        elif kind == 1:
+            while True:
+                v = r(32 * nb) | (1 << (32 * nb - 1))
+                v2 = v >> (32 * (nb - 2))
+                r1 = (int(rng.integers(2, B - 1)) * v2 - 1) * B ** (nb - 3) + r(32 * (nb - 3))
+                if r1 < v:
+                    break
+            u0 = r(32)
+        else:
+            v = B ** nb - 1 - r(int(rng.choice([0, 1, 8, 32])))
+            r1 = v - 1 - r(int(rng.choice([0, 1, 2, 40])))
+            u0 = r(32)
+        q1 = int(rng.integers(1, B))
+        u = ((q1 * v + r1) * B + u0) * B + r(32)
+        assert u < B ** (nb + 3)
+        a[i] = O.from_int(u, nb + 3)
+        b[i] = O.from_int(v, nb)
+    return a, b
+
+
+@pytest.mark.parametrize("nb", [32, 64, 96, 128])
+def test_div_later_digits_two_lanes(handle, nb):
+    """Every two-lane divisor size (16-, 32-, 48- and 64-limb halves) on the
+    constructed later-digit cases, against GMP."""
+    n = 384
+    a, b = _later_digit_cases(nb, n, 2000 + nb)
+    A, B = handle.from_array(a), handle.from_array(b)
+    Q, R = handle.integers(32 * (nb + 3), n), handle.integers(32 * nb, n)
+    handle.divmod(Q, R, A, B)
+    wq, wr = gmp(O.GO_DIV, n, nb + 3, a, b, out2_limbs=nb)
+    assert_rows_equal(Q.get(), wq, "q, %d-limb divisors" % nb)
+    assert_rows_equal(R.get(), wr, "r, %d-limb divisors" % nb)
