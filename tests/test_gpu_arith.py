@@ -246,6 +246,25 @@ def test_mul_group_width_by_batch(handle, bits, count):
                       "%d-bit squares x %d" % (bits, count))
 
 
+@pytest.mark.parametrize("bits", [1024, 2048])
+def test_mul_cols_big_batch_ragged_end(handle, bits):
+    """Batches of 2^18 products and more (squares, and products up to 1024
+    bits) run in 256-lane workgroups in XCD order: 262,244 of them end in a
+    partial wave and a workgroup whose last waves are past the batch;
+    products and squares against GMP on a sample that includes the last
+    rows."""
+    n, limbs = (1 << 18) + 100, bits // 32
+    a = inputs.uniform(4100 + bits, n, limbs)
+    b = inputs.uniform(4200 + bits, n, limbs)
+    A, B = handle.from_array(a), handle.from_array(b)
+    P = handle.integers(2 * bits, n)
+    rows = np.unique(np.concatenate([np.linspace(0, n - 1, 400).astype(int), np.arange(n - 130, n)]))
+    handle.mul(P, A, B)
+    assert_rows_equal(P.get()[rows], gmp(O.GO_MUL, rows.size, 2 * limbs, a[rows], b[rows]), "products")
+    handle.mul(P, A, A)
+    assert_rows_equal(P.get()[rows], gmp(O.GO_MUL, rows.size, 2 * limbs, a[rows], a[rows]), "squares")
+
+
 def test_mul_closed_form_and_square(handle):
     """(2^k - 1)^2 = 2^2k - 2^(k+1) + 1, and mul(a, a) takes the square path."""
     n = 64
