@@ -41,7 +41,7 @@ VALU_PEAK_TOPS = 78.6
 # by window width, from --pmc runs of `bench.py --only powm`: reads are
 # 32/64/128 x TCC_EA0_RDREQ_{32B,64B,128B} (exact on a copy kernel of known
 # size), writes are WRITE_SIZE (exact on the same copy)
-POWM_TRAFFIC_BYTES = {(2, 37, 28, 6): 9053696320 + 1445110656,
+POWM_TRAFFIC_BYTES = {(2, 37, 28, 6): 9053542464 + 1444922912,
                       (2, 37, 28, 7): 7917544640 + 2808747328}
 
 
