@@ -606,10 +606,11 @@ hipError_t launch_sqr(OutOperand p, Operand a, uint32_t count, hipStream_t strea
 // product stays at h limbs (within the row kernels up to 1,024-limb
 // operands) and the carry terms join the others.  Three half-size products
 // of the whole batch (recursively, until they fit the row kernels), two sums
-// and one pass that adds the shifted terms with a signed carry.  When b has
-// no more than h limbs, a alone is split: a b = (a1 b) B + a0 b.  The terms
-// live in a workspace, instance i in column i of rows of stride
-// round_up(count, 64).
+// and a recombination that adds the shifted terms with a signed carry (the
+// sums run on it too; each is a chunk pass and a carry pass).  When b has
+// no more than h limbs, a alone is split: a b = (a1 b) B + a0 b.  The terms,
+// and the chunks' carry-outs, live in a workspace, instance i in column i of
+// rows of stride round_up(count, 64).
 
 namespace {
 
