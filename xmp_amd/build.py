@@ -48,7 +48,9 @@ COMMON = [
 # MI355X, the 65,536 x 2048-bit powm launch took 48.5-48.7 ms against
 # 49.4-49.7 ms with the default occupancy-oriented scheduler (the kernel runs
 # at its register-limited occupancy of two waves per SIMD either way).  The
-# few-digit latency layouts (k_powm_latency.hip) keep the default scheduler.
+# few-digit latency layouts (k_powm_latency.hip) keep the default scheduler,
+# and so does k_arith.hip, whose build the ILP scheduler crashes (the ROCm 7.2
+# compiler ends with a segmentation fault).
 DEVICE_OPTS = {
     "k_powm.hip": ["-Xarch_device", "-mllvm=-misched=gcn-iterative-ilp"],
 }
