@@ -9,8 +9,9 @@ and zero-padded outputs, all-ones operands (every term and the signed carry
 of the recombination at their extremes), the one term that reaches past the
 product (ca cb B^2 at limb 3h = len(a) + len(b)) with the product's end on a
 chunk boundary of the recombination, squares, index arrays with a
-broadcast multiplier, in place, and the schoolbook fallback when the scratch
-limit leaves no room for the workspace.
+broadcast multiplier, in place, the schoolbook fallback when the scratch
+limit leaves no room for the workspace, and the bench's shape (4,096 pairs of
+32,768 bits, whose half products take the 16-lane row groups).
 """
 import numpy as np
 import pytest
@@ -124,3 +125,20 @@ def test_karatsuba_scratch_limit_falls_back(fresh_handle):
     finally:
         h.set_scratch_limit(0)
     assert_rows_equal(P.get(), gmp(O.GO_MUL, n, 2 * bits // 32, a, b), "schoolbook fallback")
+
+
+def test_karatsuba_bench_shape(handle):
+    """4,096 products and squares of 32,768 bits, the bench's shape: the
+    half products then take the 16-lane row groups (below 4,096 products they
+    take 32-lane ones), against GMP on a sample including the last rows."""
+    n, limbs = 4096, 1024
+    a, b = _pair(5151, n, limbs, limbs)
+    a[0] = 0xFFFFFFFF
+    b[0] = 0xFFFFFFFF
+    A, B = handle.from_array(a), handle.from_array(b)
+    P = handle.integers(2 * 32 * limbs, n)
+    rows = np.unique(np.concatenate([np.linspace(0, n - 1, 200).astype(int), np.arange(n - 70, n)]))
+    handle.mul(P, A, B)
+    assert_rows_equal(P.get()[rows], gmp(O.GO_MUL, rows.size, 2 * limbs, a[rows], b[rows]), "products")
+    handle.mul(P, A, A)
+    assert_rows_equal(P.get()[rows], gmp(O.GO_MUL, rows.size, 2 * limbs, a[rows], a[rows]), "squares")
