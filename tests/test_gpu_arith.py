@@ -265,6 +265,53 @@ def test_mul_cols_big_batch_ragged_end(handle, bits):
     assert_rows_equal(P.get()[rows], gmp(O.GO_MUL, rows.size, 2 * limbs, a[rows], a[rows]), "squares")
 
 
+@pytest.mark.parametrize("bits,count,bits_p", [
+    (96, 130, 192), (256, 130, 512), (512, 130, 1024), (512, 130, 704),     # 1x5, 1x10, 1x19 (truncated)
+    (1536, 130, 3072), (3072, 130, 6144), (4096, 130, 4096),                # 2x28, 4x37 (truncated)
+    (6144, 130, 12288), (8192, 4096, 16384), (8192, 130, 16416),            # 32x10, 8x38, 32x10 (padded)
+    (12288, 130, 24576), (16384, 4096, 32768), (16384, 130, 20000)])        # 32x19, 16x38, 32x19 (truncated)
+def test_sqr_lane_groups(handle, bits, count, bits_p):
+    """Squares on the row kernels: the one-lane layouts (up to 532 bits)
+    take each product of two different digits once (doubled) or once in
+    each order, by a tournament on the digits' places, and the lane groups
+    run full rows.  Every layout, truncated and padded outputs, all-ones rows
+    (the column sums at their bound), against GMP."""
+    limbs = bits // 32
+    a = inputs.uniform(5300 + bits + count, count, limbs)
+    a[:2] = 0xFFFFFFFF
+    a[2, :] = 0
+    a[2, -1] = 0xFFFFFFFF
+    A = handle.from_array(a)
+    P = handle.integers(bits_p, count)
+    handle.mul(P, A, A)
+    rows = np.unique(np.concatenate([np.arange(4), np.linspace(0, count - 1, 120).astype(int)]))
+    assert_rows_equal(P.get()[rows], gmp(O.GO_MUL, rows.size, bits_p // 32, a[rows], a[rows]),
+                      "%d-bit squares x %d -> %d bits" % (bits, count, bits_p))
+
+
+@pytest.mark.parametrize("bits", [256, 4096, 16384])
+def test_sqr_lane_groups_indexed(handle, bits):
+    """xmpIntegersSqrAsync with an input gather (slot 1) and an output
+    permutation (slot 0) on the row kernels' squares."""
+    n, limbs = 200, bits // 32
+    a = inputs.uniform(5400 + bits, 50, limbs)
+    rng = np.random.default_rng(bits)
+    ia = rng.integers(0, 50, n).astype(np.uint32)
+    ip = rng.permutation(n).astype(np.uint32)
+    A = handle.from_array(a)
+    P = handle.integers(2 * bits, n)
+    handle.set_indices(0, ip)
+    handle.set_indices(1, ia)
+    try:
+        assert xmp_amd.lib().xmpIntegersSqrAsync(handle.h, P.x, A.x, n) == 0
+    finally:
+        handle.clear_indices()
+    rows = gmp(O.GO_MUL, n, 2 * limbs, a[ia], a[ia])
+    want = np.zeros_like(rows)
+    want[ip] = rows
+    assert_rows_equal(P.get(), want, "indexed %d-bit squares" % bits)
+
+
 def test_mul_closed_form_and_square(handle):
     """(2^k - 1)^2 = 2^2k - 2^(k+1) + 1, and mul(a, a) takes the square path."""
     n = 64

@@ -162,6 +162,70 @@ struct MulRow {
     t[0] += f >> D;
     return lo;
   }
+  // For a square, row i = li K + KI (the digit a_i) takes column j = l K + J
+  // (lane l's digit J) once if J == KI, and twice or not at all otherwise,
+  // as a tournament on the within-lane indices decides: of a pair i != j,
+  // exactly one of row i and row j takes 2 a_i a_j (when J != KI), or both
+  // take it once (J == KI).  Both rows come before the row that retires
+  // position i + j, so every position holds the plain product's total when
+  // it is retired: the same digits, carries and accumulator bound.
+  static constexpr bool beats(int x, int y) {
+    const int dd = (y - x + K) % K;
+    return 2 * dd < K || (2 * dd == K && x < K / 2);
+  }
+  // up to 10 digits per lane keep their doubles in registers; else a row
+  // doubles its multiplier digit once
+  static constexpr bool kOwnDoubled = K <= 10;
+  template <int KI>
+  __device__ __forceinline__ uint32_t retire_sq(uint64_t (&t)[K], const uint32_t (&da)[K], const uint32_t (&a2)[K],
+                                                uint32_t d) const {
+    const uint32_t d2 = kOwnDoubled ? 0u : d << 1;
+    auto twice = [&](int j) { return kOwnDoubled ? (uint64_t)a2[j] * d : (uint64_t)da[j] * d2; };
+    uint64_t f = t[0];
+    if constexpr (KI == 0) f += (uint64_t)da[0] * d;
+    else if constexpr (beats(KI, 0)) f += twice(0);
+    const uint32_t lo = (uint32_t)f & MASK;
+    const uint32_t up = g.from_next(lo);
+    sq_cols<KI>(t, da, a2, d, d2, std::make_integer_sequence<int, K - 1>());
+    t[K - 1] = up;
+    t[0] += f >> D;
+    return lo;
+  }
+  template <int KI, int... J>
+  __device__ __forceinline__ void sq_cols(uint64_t (&t)[K], const uint32_t (&da)[K], const uint32_t (&a2)[K],
+                                          uint32_t d, uint32_t d2, std::integer_sequence<int, J...>) const {
+    (sq_col<KI, J + 1>(t, da, a2, d, d2), ...);
+  }
+  template <int KI, int J>
+  __device__ __forceinline__ void sq_col(uint64_t (&t)[K], const uint32_t (&da)[K], const uint32_t (&a2)[K],
+                                         uint32_t d, uint32_t d2) const {
+    uint64_t x = t[J];
+    if constexpr (J == KI) x += (uint64_t)da[J] * d;
+    else if constexpr (beats(KI, J)) x += kOwnDoubled ? (uint64_t)a2[J] * d : (uint64_t)da[J] * d2;
+    t[J - 1] = x;
+  }
+  // pack a retired digit into the bit buffer, storing each completed limb
+  __device__ __forceinline__ void pack(uint32_t lo, const OutOperand &p, uint32_t pi, bool store) {
+    obuf |= (uint64_t)lo << onb;
+    onb += D;
+    if (onb >= 32) {
+      if (store && limb < p.len) p.data[(size_t)limb * p.stride + pi] = (uint32_t)obuf;
+      limb++;
+      obuf >>= 32;
+      onb -= 32;
+    }
+  }
+  template <int KI>
+  __device__ __forceinline__ void row_sq(uint64_t (&t)[K], const uint32_t (&da)[K], const uint32_t (&a2)[K],
+                                         uint32_t d, const OutOperand &p, uint32_t pi, bool store) {
+    pack(retire_sq<KI>(t, da, a2, d), p, pi, store);
+  }
+  template <int... KI>
+  __device__ __forceinline__ void rows_sq(uint64_t (&t)[K], const uint32_t (&da)[K], const uint32_t (&a2)[K],
+                                          const uint32_t *st, uint32_t r0, uint32_t rows, const OutOperand &p,
+                                          uint32_t pi, bool store, std::integer_sequence<int, KI...>) {
+    ((r0 + KI < rows ? row_sq<KI>(t, da, a2, st[r0 + KI], p, pi, store) : void()), ...);
+  }
   // one row with the retired digit packed into the bit buffer
   __device__ __forceinline__ void row(uint64_t (&t)[K], const uint32_t (&da)[K], uint32_t d, const OutOperand &p,
                                       uint32_t pi, bool store) {
@@ -177,7 +241,7 @@ struct MulRow {
   }
 };
 
-template <int L, int K, int D>
+template <int L, int K, int D, bool SQR = false>
 __global__ void __launch_bounds__(64, 2) mul_lanes_kernel(OutOperand p, Operand a, Operand b, uint32_t count) {
   constexpr int ND = L * K;
   constexpr int G = 64 / L;
@@ -211,15 +275,26 @@ __global__ void __launch_bounds__(64, 2) mul_lanes_kernel(OutOperand p, Operand 
   const uint32_t nout = (32u * p.len + D - 1) / D;   // digits that reach the output
   const uint32_t rows = ndb < nout ? ndb : nout;
   MulRow<L, K, D> m(g);
-  uint32_t r = 0;
-  for (; r + 4 <= rows; r += 4) {
-    const uint4 bb = *reinterpret_cast<const uint4 *>(st + r);
-    m.row(t, da, bb.x, p, pi, store);
-    m.row(t, da, bb.y, p, pi, store);
-    m.row(t, da, bb.z, p, pi, store);
-    m.row(t, da, bb.w, p, pi, store);
+  if constexpr (SQR) {
+    // b is a: rows li K .. li K + K - 1 with the tournament of their
+    // within-lane indices, lane by lane (the loop over lanes is kept)
+    uint32_t a2[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) a2[k] = MulRow<L, K, D>::kOwnDoubled ? da[k] << 1 : 0u;
+#pragma unroll 1
+    for (uint32_t r0 = 0; r0 < rows; r0 += K)
+      m.rows_sq(t, da, a2, st, r0, rows, p, pi, store, std::make_integer_sequence<int, K>());
+  } else {
+    uint32_t r = 0;
+    for (; r + 4 <= rows; r += 4) {
+      const uint4 bb = *reinterpret_cast<const uint4 *>(st + r);
+      m.row(t, da, bb.x, p, pi, store);
+      m.row(t, da, bb.y, p, pi, store);
+      m.row(t, da, bb.z, p, pi, store);
+      m.row(t, da, bb.w, p, pi, store);
+    }
+    for (; r < rows; r++) m.row(t, da, st[r], p, pi, store);
   }
-  for (; r < rows; r++) m.row(t, da, st[r], p, pi, store);
   if (rows < ndb) {
     // truncated: the retired digits already cover every output bit
     return;
@@ -277,10 +352,11 @@ __global__ void __launch_bounds__(64, 2) mul_lanes_kernel(OutOperand p, Operand 
   }
 }
 
-template <int L, int K, int D>
+template <int L, int K, int D, bool SQR = false>
 hipError_t launch_mul_lanes(OutOperand p, Operand a, Operand b, uint32_t count, hipStream_t stream) {
   constexpr uint32_t G = 64 / L;
-  hipLaunchKernelGGL((mul_lanes_kernel<L, K, D>), dim3(div_round_up(count, G)), dim3(64), 0, stream, p, a, b, count);
+  hipLaunchKernelGGL((mul_lanes_kernel<L, K, D, SQR>), dim3(div_round_up(count, G)), dim3(64), 0, stream, p, a, b,
+                     count);
   return hipGetLastError();
 }
 
@@ -549,6 +625,47 @@ hipError_t launch_sub(OutOperand d, Operand a, Operand b, uint32_t count, hipStr
   return hipGetLastError();
 }
 
+namespace {
+
+// register-resident row kernels, the longer operand (a) as the multiplicand:
+// one lane per instance up to 1024 bits, then lane groups; b.len <= a.len
+// keeps the multiplier within the accumulator bound.  SQR: b is a, and on
+// the one-lane layouts (up to 532 bits) each row takes the symmetric half of
+// its products.  The lane groups keep the full rows for squares too:
+// measured A/B on one MI355X, the symmetric rows made 512-bit squares 7 %
+// faster (0.035 against 0.038 ms for 2^20) but groups of 4 to 32 lanes 2-7 %
+// slower (16384 bits x 4,096: 0.131 against 0.125 ms), whose rows wait on
+// the retire-and-hand-over chain rather than on their multiply-adds.  False
+// when a is longer than every group holds.
+template <bool SQR>
+bool launch_rows(OutOperand p, Operand a, Operand b, uint32_t count, hipStream_t stream, hipError_t *e) {
+  if (a.len <= 4) *e = launch_mul_lanes<1, 5, 28, SQR>(p, a, b, count, stream);          // 140 bits
+  else if (a.len <= 8) *e = launch_mul_lanes<1, 10, 28, SQR>(p, a, b, count, stream);    // 280 bits
+  else if (a.len <= 16) *e = launch_mul_lanes<1, 19, 28, SQR>(p, a, b, count, stream);   // 532 bits
+  else if (a.len <= 32) *e = launch_mul_lanes<1, 37, 28>(p, a, b, count, stream);        // 1036 bits
+  else if (a.len <= 48) *e = launch_mul_lanes<2, 28, 28>(p, a, b, count, stream);        // 56 digits >= 1536 bits
+  else if (a.len <= 64) *e = launch_mul_lanes<2, 37, 28>(p, a, b, count, stream);        // 74 digits >= 2048 bits
+  else if (a.len <= 128) *e = launch_mul_lanes<4, 37, 28>(p, a, b, count, stream);       // 148 digits >= 4096 bits
+  // Above 4096 bits a batch of fewer than 4,096 products takes 32-lane groups
+  // (count / 2 waves where 8 or 16 lanes leave SIMDs without a wave).
+  // Measured on one MI355X: 1,024 x 16384-bit products 0.082 against 0.100
+  // ms, 2,048 0.091 against 0.110, 1,024 x 8192-bit 0.038 against 0.048; at
+  // 4,096 and above the extra lanes' per-row moves cost more than the waves
+  // gain (16384 bits x 8,191: 0.23 against 0.20 ms; 16 x 19 digits at 8192
+  // bits was slower than 8 x 38 at every batch from 8,191 to 16,383).
+  else if (a.len <= 256)   // 304 digits >= 8192 bits
+    *e = count < 4096 ? launch_mul_lanes<32, 10, 27>(p, a, b, count, stream)
+                      : launch_mul_lanes<8, 38, 27>(p, a, b, count, stream);
+  else if (a.len <= 512)   // 608 digits >= 16384 bits
+    *e = count < 4096 ? launch_mul_lanes<32, 19, 27>(p, a, b, count, stream)
+                      : launch_mul_lanes<16, 38, 27>(p, a, b, count, stream);
+  else
+    return false;
+  return true;
+}
+
+}  // namespace
+
 hipError_t launch_mul(OutOperand p, Operand a, Operand b, uint32_t count, hipStream_t stream) {
   // one lane per product with both operands in registers (product scanning)
   // for 513-1024-bit by 513-1024-bit and 1537-2048-bit by 513-2048-bit
@@ -557,31 +674,8 @@ hipError_t launch_mul(OutOperand p, Operand a, Operand b, uint32_t count, hipStr
     if (b.len <= 32) return launch_mul_cols<74, 37, false>(p, a, b, count, stream);
     return launch_mul_cols<74, 74, false>(p, a, b, count, stream);
   }
-  // register-resident row kernels, the longer operand (a) as the multiplicand:
-  // one lane per instance up to 1024 bits, then lane groups; b.len <= a.len
-  // keeps the multiplier within the accumulator bound
-  if (a.len <= 4) return launch_mul_lanes<1, 5, 28>(p, a, b, count, stream);      // 140 bits
-  if (a.len <= 8) return launch_mul_lanes<1, 10, 28>(p, a, b, count, stream);     // 280 bits
-  if (a.len <= 16) return launch_mul_lanes<1, 19, 28>(p, a, b, count, stream);    // 532 bits
-  if (a.len <= 32) return launch_mul_lanes<1, 37, 28>(p, a, b, count, stream);   // 1036 bits
-  if (a.len <= 48) return launch_mul_lanes<2, 28, 28>(p, a, b, count, stream);    // 56 digits >= 1536 bits
-  if (a.len <= 64) return launch_mul_lanes<2, 37, 28>(p, a, b, count, stream);    // 74 digits >= 2048 bits
-  if (a.len <= 128) return launch_mul_lanes<4, 37, 28>(p, a, b, count, stream);   // 148 digits >= 4096 bits
-  // Above 4096 bits a batch of fewer than 4,096 products takes 32-lane groups
-  // (count / 2 waves where 8 or 16 lanes leave SIMDs without a wave).
-  // Measured on one MI355X: 1,024 x 16384-bit products 0.082 against 0.100
-  // ms, 2,048 0.091 against 0.110, 1,024 x 8192-bit 0.038 against 0.048; at
-  // 4,096 and above the extra lanes' per-row moves cost more than the waves
-  // gain (16384 bits x 8,191: 0.23 against 0.20 ms; 16 x 19 digits at 8192
-  // bits was slower than 8 x 38 at every batch from 8,191 to 16,383).
-  if (a.len <= 256) {   // 304 digits >= 8192 bits
-    if (count < 4096) return launch_mul_lanes<32, 10, 27>(p, a, b, count, stream);
-    return launch_mul_lanes<8, 38, 27>(p, a, b, count, stream);
-  }
-  if (a.len <= 512) {   // 608 digits >= 16384 bits
-    if (count < 4096) return launch_mul_lanes<32, 19, 27>(p, a, b, count, stream);
-    return launch_mul_lanes<16, 38, 27>(p, a, b, count, stream);
-  }
+  hipError_t e;
+  if (launch_rows<false>(p, a, b, count, stream, &e)) return e;
   hipLaunchKernelGGL(mul_general_kernel, dim3(div_round_up(count, 256)), dim3(256), 0, stream, p, a, b, count);
   return hipGetLastError();
 }
@@ -590,6 +684,9 @@ hipError_t launch_sqr(OutOperand p, Operand a, uint32_t count, hipStream_t strea
   // symmetric product scanning: each product of two different digits once
   if (a.len > 16 && a.len <= 32) return launch_mul_cols<37, 37, true>(p, a, a, count, stream);
   if (a.len > 48 && a.len <= 64) return launch_mul_cols<74, 74, true>(p, a, a, count, stream);
+  // symmetric rows on the one-lane layouts, full rows on the lane groups
+  hipError_t e;
+  if (launch_rows<true>(p, a, a, count, stream, &e)) return e;
   return launch_mul(p, a, a, count, stream);
 }
 
