@@ -271,9 +271,9 @@ def test_mul_cols_big_batch_ragged_end(handle, bits):
     (6144, 130, 12288), (8192, 4096, 16384), (8192, 130, 16416),            # 32x10, 8x38, 32x10 (padded)
     (12288, 130, 24576), (16384, 4096, 32768), (16384, 130, 20000)])        # 32x19, 16x38, 32x19 (truncated)
 def test_sqr_lane_groups(handle, bits, count, bits_p):
-    """Squares on the row kernels: the one-lane layouts (up to 532 bits)
-    take each product of two different digits once (doubled) or once in
-    each order, by a tournament on the digits' places, and the lane groups
+    """Squares on the row kernels: the one-lane and 32-lane layouts take
+    each product of two different digits once (doubled) or once in each
+    order, by a tournament on the digits' places, and the other lane groups
     run full rows.  Every layout, truncated and padded outputs, all-ones rows
     (the column sums at their bound), against GMP."""
     limbs = bits // 32

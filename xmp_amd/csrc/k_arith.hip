@@ -224,7 +224,11 @@ struct MulRow {
   __device__ __forceinline__ void rows_sq(uint64_t (&t)[K], const uint32_t (&da)[K], const uint32_t (&a2)[K],
                                           const uint32_t *st, uint32_t r0, uint32_t rows, const OutOperand &p,
                                           uint32_t pi, bool store, std::integer_sequence<int, KI...>) {
-    ((r0 + KI < rows ? row_sq<KI>(t, da, a2, st[r0 + KI], p, pi, store) : void()), ...);
+    // the block's digits first, unconditionally (r0 + KI < ceil(rows/K) K
+    // <= L K, within the stage), so no row waits on its own LDS load
+    uint32_t dd[K];
+    ((dd[KI] = st[r0 + KI]), ...);
+    ((r0 + KI < rows ? row_sq<KI>(t, da, a2, dd[KI], p, pi, store) : void()), ...);
   }
   // one row with the retired digit packed into the bit buffer
   __device__ __forceinline__ void row(uint64_t (&t)[K], const uint32_t (&da)[K], uint32_t d, const OutOperand &p,
@@ -630,13 +634,13 @@ namespace {
 // register-resident row kernels, the longer operand (a) as the multiplicand:
 // one lane per instance up to 1024 bits, then lane groups; b.len <= a.len
 // keeps the multiplier within the accumulator bound.  SQR: b is a, and on
-// the one-lane layouts (up to 532 bits) each row takes the symmetric half of
-// its products.  The lane groups keep the full rows for squares too:
-// measured A/B on one MI355X, the symmetric rows made 512-bit squares 7 %
-// faster (0.035 against 0.038 ms for 2^20) but groups of 4 to 32 lanes 2-7 %
-// slower (16384 bits x 4,096: 0.131 against 0.125 ms), whose rows wait on
-// the retire-and-hand-over chain rather than on their multiply-adds.  False
-// when a is longer than every group holds.
+// the one-lane and 32-lane layouts each row takes the symmetric half of its
+// products.  Measured A/B on one MI355X: 2^20 512-bit squares 0.035 against
+// 0.038 ms, 1,024 16384-bit ones (32 x 19) 0.076 against 0.081, and the
+// 65536-bit Karatsuba squares whose half products they are 0.87 against
+// 0.91; the 2-, 4-, 8- and 16-lane groups keep full rows (4 x 37: 0.093
+// against 0.087 ms for 65,536 4096-bit squares; 8 x 38 and 16 x 38 even).
+// False when a is longer than every group holds.
 template <bool SQR>
 bool launch_rows(OutOperand p, Operand a, Operand b, uint32_t count, hipStream_t stream, hipError_t *e) {
   if (a.len <= 4) *e = launch_mul_lanes<1, 5, 28, SQR>(p, a, b, count, stream);          // 140 bits
@@ -654,10 +658,10 @@ bool launch_rows(OutOperand p, Operand a, Operand b, uint32_t count, hipStream_t
   // gain (16384 bits x 8,191: 0.23 against 0.20 ms; 16 x 19 digits at 8192
   // bits was slower than 8 x 38 at every batch from 8,191 to 16,383).
   else if (a.len <= 256)   // 304 digits >= 8192 bits
-    *e = count < 4096 ? launch_mul_lanes<32, 10, 27>(p, a, b, count, stream)
+    *e = count < 4096 ? launch_mul_lanes<32, 10, 27, SQR>(p, a, b, count, stream)
                       : launch_mul_lanes<8, 38, 27>(p, a, b, count, stream);
   else if (a.len <= 512)   // 608 digits >= 16384 bits
-    *e = count < 4096 ? launch_mul_lanes<32, 19, 27>(p, a, b, count, stream)
+    *e = count < 4096 ? launch_mul_lanes<32, 19, 27, SQR>(p, a, b, count, stream)
                       : launch_mul_lanes<16, 38, 27>(p, a, b, count, stream);
   else
     return false;
@@ -684,7 +688,7 @@ hipError_t launch_sqr(OutOperand p, Operand a, uint32_t count, hipStream_t strea
   // symmetric product scanning: each product of two different digits once
   if (a.len > 16 && a.len <= 32) return launch_mul_cols<37, 37, true>(p, a, a, count, stream);
   if (a.len > 48 && a.len <= 64) return launch_mul_cols<74, 74, true>(p, a, a, count, stream);
-  // symmetric rows on the one-lane layouts, full rows on the lane groups
+  // symmetric rows on the one-lane and 32-lane layouts, full rows elsewhere
   hipError_t e;
   if (launch_rows<true>(p, a, a, count, stream, &e)) return e;
   return launch_mul(p, a, a, count, stream);
